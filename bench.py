@@ -1,0 +1,203 @@
+"""Benchmark: MP-PCA voxel-frames/s on the 80x80x48x300 workload, one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-stride S] [--no-cpu]
+
+A step is one full ``pdamd_denoise`` call over the volume (selection, Gram +
+tridiagonalisation, eigen/threshold, synthesis + recombination,
+normalisation) with the input already resident in HBM; the timed region is
+bracketed by ``torch.cuda.synchronize()``.  Per-stage kernel times are taken
+live from HIP events recorded on the work stream during the timed steps.
+
+Prints one JSON line with ``roofline`` (dominant kernel) and ``cpu_baseline``
+(the CPU oracle, single-threaded, on a strided sample of the same patches,
+extrapolated to the full volume).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (REPO, os.path.join(REPO, "patch-denoising_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+WORKLOAD = "mppca_80x80x48x300_p5_o4_ellipsoid_weighted"
+
+
+def algorithmic_work(n, N, M, T, ranks):
+    """Per-launch algorithmic work of each stage for the selected patches.
+
+    gram_tridiag: Gram n(n+1)N (SYRK count, min side) + Householder
+                  tridiagonalisation 4/3 n^3, real flops per patch.
+    synth:        4 M T r flops per patch (two rank-r products).
+    Bytes (HBM, ideal): input read once + output and weights written once.
+    """
+    P = len(ranks)
+    f_gram = P * (n * (n + 1) * N + 4.0 / 3.0 * n**3)
+    f_syn = float(4.0 * M * T * np.sum(ranks))
+    return f_gram, f_syn
+
+
+def cpu_baseline(data, mask, ps, po, sel, stride):
+    """Time the CPU oracle (single thread) on every ``stride``-th selected patch."""
+    from oracle import cpu_ref as O
+
+    src = O.StridedPatches(data, ps, po)
+    out = np.zeros(data.shape, np.float32)
+    w = np.zeros(data.shape, np.float32)
+    dst, wts = O.StridedPatches(out, ps, po), O.StridedPatches(w, ps, po)
+    sample = sel[::stride]
+    M = int(np.prod(ps[:-1]))
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    with threadpool_limits(limits=1):
+        blas = [(d.get("internal_api"), d.get("num_threads")) for d in threadpool_info()]
+        t0 = time.perf_counter()
+        for i in sample:
+            X = src.get(i).reshape(M, -1)
+            P, r, _ = O.mppca_patch(X, 1.0)
+            th = 1 / (2 + r)
+            o = dst.get(i)
+            o += P.reshape(ps) * th
+            ww = wts.get(i)
+            ww += th
+        dt = time.perf_counter() - t0
+    per_patch = dt / len(sample)
+    full = per_patch * len(sel)
+    return dict(sample_patches=int(len(sample)), seconds=dt, per_patch_s=per_patch,
+                full_estimate_s=full, blas=blas)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="auto")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU sharding is described in DESIGN.md, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from patch_denoise import _native as N
+    from patch_denoise import workloads
+    from patch_denoise.space_time.base import _geometry
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t_gen = time.perf_counter()
+    data, mask = workloads.config2()
+    gen_s = time.perf_counter() - t_gen
+    shape = data.shape
+    ps, po = (5, 5, 5, shape[-1]), (4, 4, 4, shape[-1])
+    s4, p4, st4 = _geometry(shape, ps, po)
+    prob = N.make_problem(s4, p4, st4, np.float32, N.MPPCA, "weighted", args.precision)
+    prob.threshold_scale2 = 1.0
+    prob.scale_f64 = 0
+    prob.mask_kind = N.MASK_SPATIAL
+    prob.mask_threshold = 50.0
+    dev_x = torch.from_numpy(data).cuda()
+    run = N.DeviceRun(prob, dev_x, mask, None, want_patch_info=True)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        run.run()
+    torch.cuda.synchronize()
+
+    stage_ms = (ctypes.c_float * 5)()
+    stage_n = (ctypes.c_int64 * 5)()
+    prob.stage_ms = ctypes.cast(stage_ms, ctypes.c_void_p).value
+    prob.stage_launches = ctypes.cast(stage_n, ctypes.c_void_p).value
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prob.stage_ms = None
+    prob.stage_launches = None
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    vf = float(np.prod(shape))
+    value = vf / (ms_per_step / 1e3)
+    info = run.patch_info()
+    ranks = info["rank"]
+    sel = info["index"]
+    M, T = 125, shape[-1]
+    n, Nn = min(M, T), max(M, T)
+    f_gram, f_syn = algorithmic_work(n, Nn, M, T, ranks)
+    stages = {name: dict(ms_per_step=stage_ms[i] / args.steps,
+                         launches_per_step=stage_n[i] / args.steps)
+              for i, name in enumerate(N.STAGES)}
+    dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
+    # per-launch average of the dominant kernel
+    dom_ms = stage_ms[N.STAGES.index(dom)] / max(stage_n[N.STAGES.index(dom)], 1)
+    launches = max(stage_n[N.STAGES.index(dom)] / args.steps, 1)
+    if dom == "gram_tridiag":
+        achieved = f_gram / launches / (dom_ms / 1e3) / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="gram_tridiag_kernel")
+    elif dom == "synth":
+        achieved = f_syn / launches / (dom_ms / 1e3) / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="synth_kernel")
+    else:
+        # trieig: O(n) per Sturm count; report against FP32 peak on its algorithmic flops
+        achieved = 0.0
+        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=0.0, traffic=None, kernel=dom)
+
+    cpu = None
+    if not args.no_cpu:
+        sel_idx = sel.astype(np.int64)
+        cb = cpu_baseline(data, mask, ps, po, sel_idx, args.cpu_stride)
+        cpu_vf_s = vf / cb["full_estimate_s"]
+        cpu = dict(value=cpu_vf_s, unit="voxel-frames/s", cores=1, kind="port",
+                   sample=(f"oracle MP-PCA on every {args.cpu_stride}th selected patch "
+                           f"({cb['sample_patches']} of {len(sel)} patches, {cb['seconds']:.1f} s,"
+                           f" OPENBLAS_NUM_THREADS=1), extrapolated to all selected patches"),
+                   per_patch_ms=cb["per_patch_s"] * 1e3, blas_threads=cb["blas"],
+                   host_cpus=os.cpu_count())
+
+    line = {
+        "metric": "MP-PCA voxel-frames/s at 80x80x48x300",
+        "value": value,
+        "unit": "voxel-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision != "fp64" else "f64",
+        "data": "synthetic (seeded generator, patch_denoise.workloads.config2)",
+        "config": {"workload": WORKLOAD, "shape": list(shape), "patch": 5, "overlap": 4,
+                   "selected_patches": int(len(sel)), "mask": "ellipsoid (36,36,20)",
+                   "precision": args.precision},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages": stages,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "datagen_s": gen_s,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * patch_denoise_amd.h -- C ABI of the MI355X (gfx950) space-time low-rank
+ * patch denoiser.
+ *
+ * Plain pointers and sizes only.  Every pointer named *_dev is device
+ * memory on the current HIP device; `stream` is a hipStream_t (NULL = the
+ * null stream).  All work is enqueued on that stream.  Functions return 0 on
+ * success and a negative PDAMD_E* code on error; pdamd_last_error() returns
+ * a message for the last error on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference's
+ * src/patch_denoise):
+ *   pdamd_denoise            BaseSpaceTimeDenoiser.denoise, space_time/base.py:177-270,
+ *                            with the per-method _patch_processing of
+ *                            space_time/lowrank.py:42-63 (MP-PCA), :104-119 (Hybrid),
+ *                            :170-184 (Raw SVT / NORDIC), :379-407 (Optimal)
+ *   pdamd_select_patches     patch selection loop, space_time/base.py:201-227
+ *   pdamd_workspace_size     (no counterpart: the caller allocates device scratch)
+ *   pdamd_last_error, pdamd_version  (no counterpart: error reporting)
+ *
+ * A "patch batch" of independent Casorati matrices (the plugin method
+ * _patch_processing, space_time/base.py:352-357) is expressed through
+ * pdamd_denoise with a volume of shape (B, M, 1, T), patch (1, M, 1, T),
+ * step (1, 1, 1, 1), average recombination and no mask: the output is then the
+ * denoised batch itself.
+ */
+#ifndef PATCH_DENOISE_AMD_H
+#define PATCH_DENOISE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types of the 4-D volume */
+enum { PDAMD_F32 = 0, PDAMD_F64 = 1, PDAMD_C64 = 2, PDAMD_C128 = 3 };
+
+/* threshold rules */
+enum {
+  PDAMD_MPPCA = 0,   /* lowrank.py:42-63   Marchenko-Pastur PCA on the top-10 eigenvalues */
+  PDAMD_HYBRID = 1,  /* lowrank.py:104-119 Hybrid-PCA against a prior variance map       */
+  PDAMD_HARD = 2,    /* lowrank.py:170-184 hard singular-value threshold (raw SVT, NORDIC) */
+  PDAMD_OPTIMAL = 3  /* lowrank.py:379-407 optimal shrinkage (Gavish-Donoho)             */
+};
+
+/* recombination (base.py:243-256) */
+enum { PDAMD_WEIGHTED = 0, PDAMD_AVERAGE = 1, PDAMD_CENTER = 2 };
+
+/* optimal-shrinkage losses (lowrank.py:297-303) */
+enum { PDAMD_LOSS_FRO = 0, PDAMD_LOSS_NUC = 1, PDAMD_LOSS_OPE = 2 };
+
+/* mask kinds */
+enum { PDAMD_MASK_NONE = 0, PDAMD_MASK_SPATIAL = 1, PDAMD_MASK_FULL = 2 };
+
+/* eigen-stage precision */
+enum { PDAMD_PREC_AUTO = 0, PDAMD_PREC_FP32 = 1, PDAMD_PREC_FP64 = 2 };
+
+/* error codes */
+enum {
+  PDAMD_OK = 0,
+  PDAMD_EINVAL = -1,      /* bad argument */
+  PDAMD_EUNSUPPORTED = -2,/* shape / size outside what this build handles */
+  PDAMD_EHIP = -3,        /* a HIP runtime call failed */
+  PDAMD_EWORKSPACE = -4   /* workspace too small */
+};
+
+typedef struct pdamd_problem {
+  /* volume: C order (d0, d1, d2, T); fewer spatial dims are padded with 1s in front */
+  int dtype;
+  int64_t shape[4];
+  int64_t patch[4];   /* patch shape, already clipped to the volume */
+  int64_t step[4];    /* patch_shape - overlap per dim (0 allowed) */
+
+  int method;         /* PDAMD_MPPCA ... */
+  int recombination;  /* PDAMD_WEIGHTED ... */
+  int precision;      /* PDAMD_PREC_* */
+
+  /* threshold operands.  *_f64 flags: 1 when the reference compares in
+     float64 (a strong float64 operand), 0 when it compares in the data's
+     own precision (a Python float operand).                               */
+  double threshold_scale2; /* MP-PCA: threshold_scale**2 */
+  int    scale_f64;
+  double tau;              /* HARD: threshold value * scale */
+  int    tau_f64;
+  int    loss;             /* OPTIMAL */
+  double mp_median;        /* OPTIMAL: Marchenko-Pastur median for beta */
+
+  /* per-voxel prior variance map (d0*d1*d2 float64), HYBRID and OPTIMAL
+     with a noise level; NULL otherwise */
+  const double* var_map_dev;
+
+  /* selection */
+  const uint8_t* mask_dev; /* d0*d1*d2 (SPATIAL) or d0*d1*d2*T (FULL) bytes, 0/1 */
+  int    mask_kind;
+  double mask_threshold;
+
+  /* data */
+  const void* input_dev;   /* volume, dtype */
+  void*  output_dev;       /* volume, dtype (written) */
+  float* weights_dev;      /* volume, float32 (written) */
+
+  /* optional per-selected-patch outputs (NULL to skip), length >= count */
+  int32_t* patch_index_dev; /* linear patch index, increasing */
+  int32_t* rank_dev;        /* retained rank (the value that sets the weight) */
+  float*   noise_dev;       /* noise variance estimate (NaN when the rule has none) */
+  float*   eig_dev;         /* 10 per patch: top eigenvalues, ascending (MP-PCA/Hybrid) */
+
+  /* out: number of selected patches */
+  int64_t selected;
+
+  /* optional stage timing (host arrays of PDAMD_NSTAGES, NULL to skip): when
+     set, HIP events are recorded on `stream` around every kernel launch and,
+     after the call completes, stage_ms[s] += summed elapsed milliseconds and
+     stage_launches[s] += number of launches of stage s.  Adds one stream
+     synchronisation at the end of the call.                                 */
+  float*   stage_ms;
+  int64_t* stage_launches;
+} pdamd_problem;
+
+/* stages for pdamd_problem.stage_ms */
+enum {
+  PDAMD_STAGE_SELECT = 0,   /* mask window + select + scan + compact          */
+  PDAMD_STAGE_GRAM = 1,     /* gram_tridiag_kernel: gather, centre, Gram, Householder */
+  PDAMD_STAGE_EIG = 2,      /* trieig_kernel: eigenvalues, rule, eigenvectors  */
+  PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate   */
+  PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask         */
+  PDAMD_NSTAGES = 5
+};
+
+/* Bytes of device workspace pdamd_denoise needs for this problem. */
+int pdamd_workspace_size(const pdamd_problem* prob, size_t* bytes);
+
+/* Selection -> per-patch low-rank denoising -> recombination -> normalisation.
+   `workspace_dev` must hold pdamd_workspace_size() bytes.  Synchronises the
+   stream once to read the number of selected patches.                     */
+int pdamd_denoise(pdamd_problem* prob, void* workspace_dev, size_t workspace_bytes, void* stream);
+
+/* Selection only: writes the increasing linear indices of selected patches to
+   index_dev (capacity: number of patches) and their count to *count (host).
+   Uses prob's shape/patch/step/mask fields.                               */
+int pdamd_select_patches(const pdamd_problem* prob, int32_t* index_dev, int64_t* count,
+                         void* workspace_dev, size_t workspace_bytes, void* stream);
+
+const char* pdamd_last_error(void);
+const char* pdamd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATCH_DENOISE_AMD_H */

@@ -1,0 +1,1824 @@
+// Space-time low-rank patch denoising on MI355X (gfx950).
+//
+// Pipeline for one volume (all launches on the caller's stream):
+//   1. mask_window_kernel / select_kernel / scan / compact   patch selection
+//   2. gram_tridiag_kernel   per patch: gather the Casorati matrix, centre it
+//                            (per-frame spatial mean, same summation order as
+//                            NumPy), form the min-side Gram in registers, then
+//                            Householder-tridiagonalise it in LDS
+//   3. trieig_kernel         per patch, one wavefront: multisection on the
+//                            tridiagonal (Sturm counts), the method's threshold
+//                            rule, inverse iteration for the kept components,
+//                            back-transformation to Gram eigenvectors
+//   4. synth_kernel          per patch: re-gather, project on the kept
+//                            components, scatter-accumulate theta * patch into
+//                            the output and theta into the weights
+//   5. finalize_kernel       out /= weights, masked voxels -> 0
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "patch_denoise_amd.h"
+#include "pd_common.h"
+
+using namespace pd;
+
+namespace {
+
+constexpr int kNT = 256;  // threads per block for the per-patch block kernels
+
+// ===========================================================================
+// 1. Selection (base.py:201-227)
+// ===========================================================================
+
+// cnt[v * gt + it] = number of set mask entries of voxel v inside temporal
+// window it.  A spatial mask counts pt per set voxel.
+__global__ void mask_window_kernel(const uint8_t* __restrict__ mask, int kind, int64_t nvox,
+                                   int64_t T, int gt, int st, int pt, int* __restrict__ cnt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nvox * gt) return;
+  const int64_t v = q / gt;
+  const int it = (int)(q % gt);
+  int c = 0;
+  if (kind == PDAMD_MASK_SPATIAL) {
+    c = mask[v] ? pt : 0;
+  } else {
+    const uint8_t* m = mask + v * T + (int64_t)it * st;
+    for (int t = 0; t < pt; ++t) c += m[t] ? 1 : 0;
+  }
+  cnt[q] = c;
+}
+
+// flags[L] = 100 * count / size > thr ; per-block totals for the compaction.
+// Each block covers kSelPerBlock consecutive patches.
+constexpr int kSelPerThread = 4;
+constexpr int kSelPerBlock = kNT * kSelPerThread;
+
+__global__ __launch_bounds__(kNT) void select_kernel(const int* __restrict__ cnt, Geom g,
+                                                     int64_t npatch, int kind, double thr,
+                                                     double psize, uint8_t* __restrict__ flags,
+                                                     int* __restrict__ block_tot) {
+  __shared__ int red[kNT / kWave];
+  int local = 0;
+  const int64_t base = (int64_t)blockIdx.x * kSelPerBlock + threadIdx.x * kSelPerThread;
+  for (int k = 0; k < kSelPerThread; ++k) {
+    const int64_t L = base + k;
+    if (L >= npatch) break;
+    bool sel;
+    if (kind == PDAMD_MASK_NONE) {
+      sel = 100.0 > thr;
+    } else {
+      int64_t sp;
+      int t0;
+      patch_origin(g, L, sp, t0);
+      const int it = (int)(L % g.gt);
+      int64_t c = 0;
+      for (int r = 0; r < g.M; ++r) {
+        const int64_t v = (sp + row_offset(g, r)) / g.T;
+        c += cnt[v * g.gt + it];
+      }
+      sel = (100.0 * (double)c) / psize > thr;
+    }
+    flags[L] = sel ? 1 : 0;
+    local += sel ? 1 : 0;
+  }
+  const int tot = block_sum<kNT>(local, red);
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the per-block totals (single block); total -> *count.
+__global__ __launch_bounds__(1024) void scan_kernel(int* __restrict__ tot, int nb,
+                                                    int64_t* __restrict__ count) {
+  __shared__ int buf[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    const int i = b0 + threadIdx.x;
+    const int v = i < nb ? tot[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int a = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (i < nb) tot[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = carry;
+}
+
+// Write the increasing list of selected patch indices.
+__global__ __launch_bounds__(kNT) void compact_kernel(const uint8_t* __restrict__ flags,
+                                                      int64_t npatch,
+                                                      const int* __restrict__ block_off,
+                                                      int32_t* __restrict__ out) {
+  __shared__ int wsum[kNT / kWave];
+  const int64_t base = (int64_t)blockIdx.x * kSelPerBlock + threadIdx.x * kSelPerThread;
+  int f[kSelPerThread];
+  int local = 0;
+  for (int k = 0; k < kSelPerThread; ++k) {
+    const int64_t L = base + k;
+    f[k] = (L < npatch) ? flags[L] : 0;
+    local += f[k];
+  }
+  // inclusive scan of `local` across the block
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  int incl = local;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int a = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += a;
+  }
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wid; ++w) wbase += wsum[w];
+  int pos = block_off[blockIdx.x] + wbase + incl - local;
+  for (int k = 0; k < kSelPerThread; ++k) {
+    if (f[k]) out[pos++] = (int32_t)(base + k);
+  }
+}
+
+// ===========================================================================
+// 2. Gram + Householder tridiagonalisation (utils.py:102-137 / :58-78)
+// ===========================================================================
+
+// Householder tridiagonalisation of an n x n Hermitian matrix (n <= 128)
+// held in registers.  Thread (w = tid / 64, l = tid % 64) owns the entries
+// A[r][c] for rows r = w + 4 jj (jj < RJ) and columns c in {l, l + 64}.  The
+// reduction follows LAPACK ?hetd2 (lower): T = Q^H A Q with
+// Q = H_0 H_1 ... H_{n-2}, H_k = I - tau_k v_k v_k^H.  Per step only the
+// reflector and the vectors p / w travel through LDS (X, V, W, P), indexed by
+// absolute row / column with zeros at indices <= k, so the rank-2 update
+// needs no masking; rows <= k are skipped with wave-uniform branches.
+// Outputs: dd / ee (real tridiagonal), tt (tau_k), and v_k written to row k
+// of Hrow in LDS (row stride n, columns k+1 .. n-1, v_k[0] = 1).
+template <class E, int RJ>
+__device__ void tridiag_reg(E (&a)[RJ][2], int n, typename real_of<E>::type* dd,
+                            typename real_of<E>::type* ee, E* tt, E* X, E* V, E* W, E* P,
+                            E* Hrow) {
+  using R = typename real_of<E>::type;
+  constexpr bool TWO = RJ * 4 > 64;  // second column set exists
+  const int tid = threadIdx.x, l = tid & (kWave - 1), w = tid / kWave;
+  const E zero = E(R(0));
+  for (int k = 0; k < n - 1; ++k) {
+    // (a) owners of column k publish A[r][k] (r > k) and the diagonal A[k][k]
+    if (l == (k & (kWave - 1))) {
+      const bool hi = k >= kWave;
+#pragma unroll
+      for (int jj = 0; jj < RJ; ++jj) {
+        const int r = w + 4 * jj;
+        if (r >= k && r < n) {
+          const E v = hi ? a[jj][TWO ? 1 : 0] : a[jj][0];
+          if (r == k) dd[k] = re_(v);
+          else X[r] = v;
+        }
+      }
+    }
+    lds_barrier();
+    // (b) wave 0: reflector from x = X[k+1 .. n-1]
+    if (w == 0) {
+      const int r0 = k + 1 + l, r1 = k + 1 + l + kWave;
+      const E x0 = r0 < n ? X[r0] : zero;
+      const E x1 = r1 < n ? X[r1] : zero;
+      const R xn2 = wave_sum((l > 0 ? abs2_(x0) : R(0)) + abs2_(x1));
+      const E alpha = X[k + 1];
+      const R ar = re_(alpha), ai = im_(alpha);
+      const bool refl = (xn2 > R(0)) || (ai != R(0));
+      R beta = ar;
+      E tk = zero, scal = zero;
+      if (refl) {
+        beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
+        tk = mk<E>((beta - ar) / beta, -ai / beta);
+        scal = recip_(alpha - E(beta));
+      }
+      // V over absolute indices: 0 for r <= k, 1 at k + 1, x_r * scal above
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = l + h * kWave;
+        E v = zero;
+        if (refl && r > k && r < n) v = (r == k + 1) ? E(R(1)) : X[r] * scal;
+        V[r] = v;
+        if (r > k && r < n) Hrow[(size_t)k * n + r] = v;
+      }
+      if (l == 0) {
+        tt[k] = tk;
+        ee[k] = beta;
+      }
+    }
+    lds_barrier();
+    const E tk = tt[k];
+    if (re_(tk) == R(0) && im_(tk) == R(0)) continue;  // H_k = I
+    // (c) partial p_c = sum over my rows r of conj(A[r][c]) v_r.  V is zero at
+    //     rows <= k and >= n, so only whole quarters of inactive rows are skipped.
+    E p0 = zero, p1 = zero;
+    constexpr int QJ = RJ / 4;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      if (w + 4 * (qq * QJ + QJ - 1) > k) {
+#pragma unroll
+        for (int u = 0; u < QJ; ++u) {
+          const int jj = qq * QJ + u;
+          const E vr = V[w + 4 * jj];
+          p0 += conj_(a[jj][0]) * vr;
+          if (TWO) p1 += conj_(a[jj][1]) * vr;
+        }
+      }
+    }
+    P[w * 128 + l] = p0;
+    P[w * 128 + l + kWave] = p1;
+    lds_barrier();
+    // (d) wave 0: p = tau * sum_w P, K = -tau/2 p^H v, W = p + K v
+    if (w == 0) {
+      const int c0 = l, c1 = l + kWave;
+      E q0 = P[c0] + P[128 + c0] + P[256 + c0] + P[384 + c0];
+      E q1 = P[c1] + P[128 + c1] + P[256 + c1] + P[384 + c1];
+      q0 = (c0 > k && c0 < n) ? tk * q0 : zero;
+      q1 = (c1 > k && c1 < n) ? tk * q1 : zero;
+      const E v0 = V[c0], v1 = V[c1];
+      const E pv = wave_sum(conj_(q0) * v0 + conj_(q1) * v1);
+      const E kc = (R(-0.5) * tk) * pv;
+      W[c0] = q0 + kc * v0;
+      W[c1] = q1 + kc * v1;
+    }
+    lds_barrier();
+    // (e) A -= v w^H + w v^H on my entries (zeros in V / W mask the rest)
+    const E vc0 = V[l], wc0 = W[l];
+    const E vc1 = V[l + kWave], wc1 = W[l + kWave];
+    const bool set0 = k < kWave - 1;  // some column l > k exists
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      if (w + 4 * (qq * QJ + QJ - 1) > k) {
+#pragma unroll
+        for (int u = 0; u < QJ; ++u) {
+          const int jj = qq * QJ + u;
+          const E vr = V[w + 4 * jj], wr = W[w + 4 * jj];
+          if (set0) a[jj][0] -= vr * conj_(wc0) + wr * conj_(vc0);
+          if (TWO) a[jj][1] -= vr * conj_(wc1) + wr * conj_(vc1);
+        }
+      }
+    }
+  }
+  // last diagonal entry
+  {
+    const int r = n - 1;
+    if (w == (r & 3) && l == (r & (kWave - 1))) {
+#pragma unroll
+      for (int jj = 0; jj < RJ; ++jj)
+        if (w + 4 * jj == r) dd[r] = re_((r >= kWave && TWO) ? a[jj][1] : a[jj][0]);
+    }
+  }
+  lds_barrier();
+}
+
+// s + a[0] + a[stride] + ... + a[(count-1) stride], added left to right
+// (NumPy's order for an axis-0 reduction); loads are issued 8 at a time.
+template <class T>
+__device__ __forceinline__ T seq_add(T s, const T* a, int count, int stride) {
+  int r = 0;
+  for (; r + 8 <= count; r += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = a[(r + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; r < count; ++r) s += a[r * stride];
+  return s;
+}
+
+struct GramArgs {
+  Geom g;
+  const int32_t* idx;  // selected patch indices (this chunk)
+  int np;              // patches in this chunk
+  int n;               // Gram size = min(M, pt)
+  int scale;           // 1: divide the Gram by (M - 1) (eigen rules)
+  void* H;             // np x n x n   Householder rows
+  void* DE;            // np x 2n      d | e (real type)
+  void* TAU;           // np x n
+};
+
+// Staging geometry: M-side stages kTC frames x (all rows); T-side stages kRC
+// rows x (all frames).
+template <class E> struct Stage { static constexpr int TC = 32, RC = 32; };
+template <> struct Stage<double> { static constexpr int TC = 8, RC = 8; };
+template <> struct Stage<cpx<float>> { static constexpr int TC = 8, RC = 8; };
+template <> struct Stage<cpx<double>> { static constexpr int TC = 8, RC = 8; };
+
+template <class T> __device__ __forceinline__ T* lds_carve(char*& p, size_t count) {
+  T* r = reinterpret_cast<T*>(p);
+  p += ((count * sizeof(T) + 15) / 16) * 16;
+  return r;
+}
+
+// Divide by a positive integer count in the element's precision (NumPy
+// true_divide; for complex values each component is divided).
+__device__ __forceinline__ float div_n(float a, int n) { return a / (float)n; }
+__device__ __forceinline__ double div_n(double a, int n) { return a / (double)n; }
+template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
+  return {a.x / (R)n, a.y / (R)n};
+}
+
+template <class IN, class E, bool TSIDE, int TILE>
+__global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict__ vol,
+                                                           GramArgs a) {
+  using R = typename real_of<E>::type;
+  constexpr int NB = 16 * TILE;  // padded Gram size
+  constexpr bool SAME = std::is_same<IN, E>::value;
+  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
+  constexpr int LDS_W = NB + 1;  // staging row length (padded)
+  const Geom& g = a.g;
+  const int n = a.n;
+  const int lda = n | 1;
+  const int tid = threadIdx.x;
+  const int ty = tid / 16, tx = tid % 16;
+
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  // Region U is used in turn by the staging buffers (Gram accumulation), the
+  // full-matrix exchange from the 16x16 accumulation tiling to the Householder
+  // row/column ownership, and the Householder vectors (row k, columns > k).
+  const size_t stage_b = ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16 +
+                         (SAME ? 0 : ((size_t)SC * LDS_W * sizeof(E) + 15) / 16 * 16);
+  const size_t full_b = (size_t)n * lda * sizeof(E);
+  char* U = cur;
+  cur += (std::max(stage_b, full_b) + 15) / 16 * 16;
+  IN* Sraw = reinterpret_cast<IN*>(U);
+  E* Sc = SAME ? reinterpret_cast<E*>(U)
+               : reinterpret_cast<E*>(U + ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16);
+  E* A = reinterpret_cast<E*>(U);   // exchange (row stride lda)
+  E* Hl = reinterpret_cast<E*>(U);  // Householder rows (row stride n)
+  IN* mean = lds_carve<IN>(cur, NB);
+  int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : NB);
+  E* Xb = lds_carve<E>(cur, 128);
+  E* V = lds_carve<E>(cur, 128);
+  E* W = lds_carve<E>(cur, 128);
+  E* P = lds_carve<E>(cur, 4 * 128);
+  R* dd = lds_carve<R>(cur, 128);
+  R* ee = lds_carve<R>(cur, 128);
+  E* tt = lds_carve<E>(cur, 128);
+  constexpr int RJ = NB / 4;
+  const int w = tid / kWave, l = tid % kWave;
+
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    int64_t sp;
+    int t0;
+    patch_origin(g, a.idx[p], sp, t0);
+    const IN* X = vol + sp + t0;
+    const int M = g.M, T = g.pt;
+
+    E acc[TILE][TILE];
+#pragma unroll
+    for (int i = 0; i < TILE; ++i)
+#pragma unroll
+      for (int j = 0; j < TILE; ++j) acc[i][j] = E(R(0));
+
+    constexpr int NL = (SC * NB + kNT - 1) / kNT;  // staged elements per thread
+    IN buf[NL];
+    if (!TSIDE) {
+      // ---- M-side: G = Xc Xc^H (M x M), frames streamed in chunks of SC.
+      // Global loads of chunk c+1 are issued before chunk c is reduced.
+      for (int r = tid; r < NB; r += kNT) roff[r] = r < M ? row_offset(g, r) : 0;
+      lds_barrier();
+      auto load = [&](int c0) {
+        const int tc = min(SC, T - c0);
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          const int q = tid + u * kNT;
+          const int t = q % SC, r = q / SC;
+          buf[u] = (q < SC * NB && t < tc && r < M) ? X[roff[r] + c0 + t] : IN(0);
+        }
+      };
+      load(0);
+      for (int c0 = 0; c0 < T; c0 += SC) {
+        const int tc = min(SC, T - c0);
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          const int q = tid + u * kNT;
+          if (q < SC * NB) Sraw[(q % SC) * LDS_W + q / SC] = buf[u];
+        }
+        lds_barrier();
+        if (c0 + SC < T) load(c0 + SC);
+        if (tid < tc)  // sequential over rows, as numpy's axis-0 reduction
+          mean[tid] = div_n(seq_add(Sraw[tid * LDS_W], Sraw + tid * LDS_W + 1, M - 1, 1), M);
+        lds_barrier();
+        for (int q = tid; q < SC * NB; q += kNT) {
+          const int t = q / NB, r = q % NB;
+          const IN v = Sraw[t * LDS_W + r];
+          Sc[t * LDS_W + r] = (t < tc && r < M) ? cvt<E>(v - mean[t]) : E(R(0));
+        }
+        lds_barrier();
+        for (int t = 0; t < tc; ++t) {
+          E xa[TILE], xb[TILE];
+#pragma unroll
+          for (int i = 0; i < TILE; ++i) xa[i] = Sc[t * LDS_W + ty + 16 * i];
+#pragma unroll
+          for (int j = 0; j < TILE; ++j) xb[j] = conj_(Sc[t * LDS_W + tx + 16 * j]);
+#pragma unroll
+          for (int i = 0; i < TILE; ++i)
+#pragma unroll
+            for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+        }
+        lds_barrier();
+      }
+    } else {
+      // ---- T-side: G = Xc^H Xc (T x T); pass 1 = per-frame mean over all rows
+      auto load = [&](int r0) {
+        const int rc = min(SC, M - r0);
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          const int q = tid + u * kNT;
+          const int r = q / NB, t = q % NB;
+          buf[u] = (q < SC * NB && r < rc && t < T) ? X[row_offset(g, r0 + r) + t] : IN(0);
+        }
+      };
+      auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          const int q = tid + u * kNT;
+          if (q < SC * NB) Sraw[(q / NB) * LDS_W + q % NB] = buf[u];
+        }
+      };
+      IN s = IN(0);
+      load(0);
+      for (int r0 = 0; r0 < M; r0 += SC) {
+        const int rc = min(SC, M - r0);
+        store();
+        lds_barrier();
+        if (r0 + SC < M) load(r0 + SC);
+        if (tid < T) {
+          if (r0 == 0) s = seq_add(Sraw[tid], Sraw + LDS_W + tid, rc - 1, LDS_W);
+          else s = seq_add(s, Sraw + tid, rc, LDS_W);
+        }
+        lds_barrier();
+      }
+      if (tid < NB) mean[tid] = tid < T ? div_n(s, M) : IN(0);
+      load(0);
+      for (int r0 = 0; r0 < M; r0 += SC) {
+        const int rc = min(SC, M - r0);
+        store();
+        lds_barrier();
+        if (r0 + SC < M) load(r0 + SC);
+        for (int q = tid; q < SC * NB; q += kNT) {
+          const int r = q / NB, t = q % NB;
+          E c = E(R(0));
+          if (r < rc && t < T) c = cvt<E>(Sraw[r * LDS_W + t] - mean[t]);
+          Sc[r * LDS_W + t] = c;
+        }
+        lds_barrier();
+        for (int r = 0; r < rc; ++r) {
+          E xa[TILE], xb[TILE];
+#pragma unroll
+          for (int i = 0; i < TILE; ++i) xa[i] = conj_(Sc[r * LDS_W + ty + 16 * i]);
+#pragma unroll
+          for (int j = 0; j < TILE; ++j) xb[j] = Sc[r * LDS_W + tx + 16 * j];
+#pragma unroll
+          for (int i = 0; i < TILE; ++i)
+#pragma unroll
+            for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+        }
+        lds_barrier();
+      }
+    }
+
+    // ---- Gram -> Householder ownership through LDS
+    //      (optionally / (M-1), as the reference's eigen path)
+#pragma unroll
+    for (int i = 0; i < TILE; ++i) {
+      const int r = ty + 16 * i;
+#pragma unroll
+      for (int j = 0; j < TILE; ++j) {
+        const int c = tx + 16 * j;
+        if (r < n && c < n) {
+          E v = acc[i][j];
+          if (a.scale) v = div_n(v, M - 1);
+          A[(size_t)r * lda + c] = v;
+        }
+      }
+    }
+    lds_barrier();
+    E areg[RJ][2];
+#pragma unroll
+    for (int jj = 0; jj < RJ; ++jj) {
+      const int r = w + 4 * jj;
+      areg[jj][0] = (r < n && l < n) ? A[(size_t)r * lda + l] : E(R(0));
+      areg[jj][1] = (RJ * 4 > 64 && r < n && l + kWave < n) ? A[(size_t)r * lda + l + kWave]
+                                                             : E(R(0));
+    }
+    lds_barrier();
+
+    tridiag_reg<E, RJ>(areg, n, dd, ee, tt, Xb, V, W, P, Hl);
+
+    // ---- Householder rows -> global
+    E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+    for (int q = tid; q < n * n; q += kNT) Hp[q] = Hl[q];
+    // ---- d | e and tau
+    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 2 * n;
+    E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
+    for (int q = tid; q < n; q += kNT) {
+      DEp[q] = dd[q];
+      DEp[n + q] = q < n - 1 ? ee[q] : R(0);
+      TAUp[q] = q < n - 1 ? tt[q] : E(R(0));
+    }
+    lds_barrier();
+  }
+}
+
+template <class IN, class E, bool TSIDE, int TILE>
+size_t gram_lds_bytes(int n) {
+  constexpr int NB = 16 * TILE;
+  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
+  constexpr int LDS_W = NB + 1;
+  constexpr bool SAME = std::is_same<IN, E>::value;
+  using R = typename real_of<E>::type;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  const int lda = n | 1;
+  const size_t stage_b = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
+  const size_t full_b = (size_t)n * lda * sizeof(E);
+  size_t s = al(std::max(stage_b, full_b));
+  s += al(NB * sizeof(IN));
+  s += al((TSIDE ? SC : NB) * sizeof(int64_t));
+  s += 3 * al(128 * sizeof(E)) + al(4 * 128 * sizeof(E));
+  s += 2 * al(128 * sizeof(R)) + al(128 * sizeof(E));
+  return s;
+}
+
+// ===========================================================================
+// 3. Tridiagonal eigen stage + threshold rules (lowrank.py)
+// ===========================================================================
+
+struct EigArgs {
+  Geom g;
+  const int32_t* idx;
+  int np;
+  int n, kmax, kb;  // Gram size, max components per patch, inverse-iteration batch
+  int method, loss;
+  int rule_f64;     // the reference's spectrum dtype is float64 (f64/c128 data)
+  int scale_f64, tau_f64;
+  double ts2, tau, mpm, beta;
+  const double* var_map;
+  const void* H;
+  const void* DE;
+  const void* TAU;
+  void* Y;           // np x n x kmax eigenvectors of the Gram (element type)
+  void* Z;           // np x n x kmax tridiagonal eigenvectors (real)
+  double* WC;        // np x kmax synthesis weights
+  int* NCOMP;        // np
+  double* THETA;     // np
+  int32_t* RANK;     // np (user array offset to chunk start) or null
+  float* NOISE;      // np or null
+  float* EIG;        // np x 10 or null
+};
+
+template <class R> struct Lim;
+template <> struct Lim<float> {
+  __device__ static float ulp() { return 5.9604645e-8f; }
+  __device__ static float tiny() { return FLT_MIN; }
+};
+template <> struct Lim<double> {
+  __device__ static double ulp() { return 1.1102230246251565e-16; }
+  __device__ static double tiny() { return DBL_MIN; }
+};
+
+__device__ __forceinline__ float fdiv_(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double fdiv_(double a, double b) { return a / b; }
+
+// Number of eigenvalues of the tridiagonal (d, e^2) strictly below x.
+template <class R>
+__device__ __forceinline__ int sturm(const R* d, const R* e2, int n, R x, R pivmin) {
+  R q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < R(0);
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - fdiv_(e2[i - 1], q);
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < R(0);
+  }
+  return c;
+}
+
+// NumPy's float mean of a[0..k) (k <= 10): 8-way pairwise blocks for k >= 8,
+// sequential below; then divided by k.  NaN for k == 0.
+template <class RT>
+__device__ RT np_mean10(const RT* a, int k) {
+  if (k <= 0) return RT(NAN);
+  RT s;
+  if (k < 8) {
+    s = RT(-0.0);
+    for (int i = 0; i < k; ++i) s += a[i];
+  } else {
+    s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    for (int i = 8; i < k; ++i) s += a[i];
+  }
+  return s / (RT)k;
+}
+
+template <class R>
+struct WaveScratch {
+  R* d;
+  R* e2;
+  R* e;
+  R* lo;
+  R* hi;
+  R* xs;
+  int* cs;
+  int* tgt;
+  R* val;
+};
+
+// Eigenvalues of the tridiagonal at ascending indices tgt[0..nt) by
+// multisection: the 64 lanes are split into nt groups of L = 64/nt points.
+template <class R>
+__device__ void multisect(WaveScratch<R>& s, int n, R pivmin, R glo, R ghi, R atol, int nt) {
+  const int lane = threadIdx.x;
+  for (int b0 = 0; b0 < nt; b0 += kWave) {
+    const int nb = min(kWave, nt - b0);
+    const int L = kWave / nb;
+    const int j = lane / L, sub = lane % L;
+    const bool active = j < nb;
+    if (active && sub == 0) {
+      s.lo[j] = glo;
+      s.hi[j] = ghi;
+    }
+    wave_sync();
+    for (int it = 0; it < 200; ++it) {
+      R lo = active ? s.lo[j] : R(0), hi = active ? s.hi[j] : R(0);
+      const R tol = atol + R(2) * Lim<R>::ulp() * fmax(fabs(lo), fabs(hi));
+      const R mid = lo + (hi - lo) * R(0.5);
+      const bool done = !active || (hi - lo) <= tol || mid <= lo || mid >= hi;
+      if (__all(done)) break;
+      R x = lo + (hi - lo) * (R)(sub + 1) / (R)(L + 1);
+      int c = 0;
+      if (active && !done) c = sturm(s.d, s.e2, n, x, pivmin);
+      s.xs[lane] = x;
+      s.cs[lane] = c;
+      wave_sync();
+      if (active && !done && sub == 0) {
+        const int tgt = s.tgt[b0 + j];
+        R nlo = lo, nhi = hi;
+        for (int q = 0; q < L; ++q) {
+          const R xq = s.xs[lane + q];
+          if (s.cs[lane + q] <= tgt) nlo = fmax(nlo, xq);
+          else nhi = fmin(nhi, xq);
+        }
+        s.lo[j] = nlo;
+        s.hi[j] = nhi;
+      }
+      wave_sync();
+    }
+    if (active && sub == 0) s.val[b0 + j] = s.lo[j] + (s.hi[j] - s.lo[j]) * R(0.5);
+    wave_sync();
+  }
+}
+
+// Solve (T - lam I) x = b for the tridiagonal (d, e) with partial pivoting;
+// b is in rhs (stride ks), the solution overwrites it.  u0/u1/u2 scratch.
+template <class R>
+__device__ void tri_solve(const R* d, const R* e, int n, R lam, R* u0, R* u1, R* u2, R* rhs,
+                          int ks, R eps_piv) {
+  R c0 = d[0] - lam, c1 = n > 1 ? e[0] : R(0), c2 = R(0), r = rhs[0];
+  for (int i = 0; i < n - 1; ++i) {
+    R n0 = e[i], n1 = d[i + 1] - lam, n2 = (i + 1 < n - 1) ? e[i + 1] : R(0);
+    R nr = rhs[(i + 1) * ks];
+    if (fabs(n0) > fabs(c0)) {
+      R t;
+      t = c0; c0 = n0; n0 = t;
+      t = c1; c1 = n1; n1 = t;
+      t = c2; c2 = n2; n2 = t;
+      t = r; r = nr; nr = t;
+    }
+    if (c0 == R(0)) c0 = eps_piv;
+    const R f = n0 / c0;
+    u0[i * ks] = c0;
+    u1[i * ks] = c1;
+    u2[i * ks] = c2;
+    rhs[i * ks] = r;
+    c0 = n1 - f * c1;
+    c1 = n2 - f * c2;
+    c2 = R(0);
+    r = nr - f * r;
+  }
+  if (c0 == R(0)) c0 = eps_piv;
+  u0[(n - 1) * ks] = c0;
+  rhs[(n - 1) * ks] = r;
+  // back substitution
+  R z2 = R(0), z1 = r / c0;
+  rhs[(n - 1) * ks] = z1;
+  for (int i = n - 2; i >= 0; --i) {
+    const R z = (rhs[i * ks] - u1[i * ks] * z1 - u2[i * ks] * z2) / u0[i * ks];
+    rhs[i * ks] = z;
+    z2 = z1;
+    z1 = z;
+  }
+}
+
+// Shrinkers of lowrank.py:250-280 on y = s / c (float64).
+__device__ double opt_x(double y, double beta) {
+  const double t = y * y - beta - 1.0;
+  const double x = sqrt(0.5 * (t + sqrt(t * t - 4.0 * beta)));
+  return x * ((y >= 1.0 + sqrt(beta)) ? 1.0 : 0.0);
+}
+__device__ double shrink(int loss, double y, double beta) {
+  if (loss == PDAMD_LOSS_OPE) {
+    const double x = opt_x(y, beta);
+    return isnan(x) ? x : fmax(x, 0.0);
+  }
+  if (loss == PDAMD_LOSS_NUC) {
+    const double x = opt_x(y, beta);
+    const double num = x * x * x * x - sqrt(beta) * x * y - beta;
+    return (isnan(num) ? num : fmax(0.0, num)) / ((x * x) * y);
+  }
+  const double t = y * y - beta - 1.0;
+  return sqrt(fmax(t * t - 4.0 * beta, 0.0) / y);
+}
+
+template <class E, class RT>
+__global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
+  using R = typename real_of<E>::type;
+  const int lane = threadIdx.x;
+  const int n = a.n, KB = a.kb;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  WaveScratch<R> s;
+  s.d = lds_carve<R>(cur, n);
+  s.e2 = lds_carve<R>(cur, n);
+  s.e = lds_carve<R>(cur, n);
+  s.lo = lds_carve<R>(cur, kWave);
+  s.hi = lds_carve<R>(cur, kWave);
+  s.xs = lds_carve<R>(cur, kWave);
+  s.cs = lds_carve<int>(cur, kWave);
+  s.tgt = lds_carve<int>(cur, n + 16);
+  s.val = lds_carve<R>(cur, n + 16);
+  R* lam = lds_carve<R>(cur, n + 16);      // eigenvalue of each component
+  int* cidx = lds_carve<int>(cur, n + 16); // ascending eigen index of each component
+  double* wc = lds_carve<double>(cur, n + 16);
+  R* u0 = lds_carve<R>(cur, (size_t)n * KB);
+  R* u1 = lds_carve<R>(cur, (size_t)n * KB);
+  R* u2 = lds_carve<R>(cur, (size_t)n * KB);
+  R* zb = lds_carve<R>(cur, (size_t)n * KB);
+  E* yb = lds_carve<E>(cur, (size_t)n * KB);
+  __shared__ int sh_i[4];
+  __shared__ double sh_d[4];
+
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    const R* DEp = reinterpret_cast<const R*>(a.DE) + (size_t)p * 2 * n;
+    for (int i = lane; i < n; i += kWave) {
+      const R dv = DEp[i], ev = DEp[n + i];
+      s.d[i] = dv;
+      s.e[i] = ev;
+      s.e2[i] = ev * ev;
+    }
+    wave_sync();
+    // Gershgorin interval
+    R glo = R(INFINITY), ghi = R(-INFINITY), emax2 = R(0);
+    for (int i = lane; i < n; i += kWave) {
+      const R off = (i > 0 ? fabs(s.e[i - 1]) : R(0)) + (i < n - 1 ? fabs(s.e[i]) : R(0));
+      glo = fmin(glo, s.d[i] - off);
+      ghi = fmax(ghi, s.d[i] + off);
+      if (i < n - 1) emax2 = fmax(emax2, s.e2[i]);
+    }
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+      glo = fmin(glo, shfl_xor_(glo, m));
+      ghi = fmax(ghi, shfl_xor_(ghi, m));
+      emax2 = fmax(emax2, shfl_xor_(emax2, m));
+    }
+    const R tnorm = fmax(fabs(glo), fabs(ghi));
+    const R pivmin = Lim<R>::tiny() * fmax(R(1), emax2);
+    const R atol = Lim<R>::ulp() * tnorm * R(0.5);
+    glo -= R(4) * Lim<R>::ulp() * tnorm + R(4) * pivmin;
+    ghi += R(4) * Lim<R>::ulp() * tnorm + R(4) * pivmin;
+
+    // prior (Hybrid: mean variance; Optimal: mean std) over the patch voxels
+    double prior = 0.0;
+    if (a.var_map) {
+      int64_t sp;
+      int t0;
+      patch_origin(a.g, a.idx[p], sp, t0);
+      double acc = 0.0;
+      for (int r = lane; r < a.g.M; r += kWave) {
+        const double v = a.var_map[(sp + row_offset(a.g, r)) / a.g.T];
+        acc += (a.method == PDAMD_OPTIMAL) ? sqrt(v) : v;
+      }
+      prior = wave_sum(acc) / (double)a.g.M;
+    }
+
+    int ncomp = 0, rank = 0;
+    float noise = NAN;
+    // ---------------- threshold rules
+    if (a.method == PDAMD_MPPCA || a.method == PDAMD_HYBRID) {
+      const int nt = min(10, n);
+      for (int j = lane; j < nt; j += kWave) s.tgt[j] = n - 1 - j;
+      wave_sync();
+      multisect(s, n, pivmin, glo, ghi, atol, nt);
+      if (lane == 0) {
+        RT ev[10];
+        for (int q = 0; q < 10; ++q) {
+          const int j = 9 - q;  // descending position
+          ev[q] = j < nt ? (RT)s.val[j] : RT(0);
+        }
+        int k = 0;
+        RT var;
+        if (a.method == PDAMD_MPPCA) {
+          // lowrank.py:45-54
+          double mv = (double)np_mean10(ev, 10) * (4.0 * sqrt((double)(10 - k + 1) / a.g.M));
+          while (k < 10 && mv < (double)(ev[9 - k] - ev[0])) {
+            ++k;
+            mv = (double)np_mean10(ev, 10 - k) * (4.0 * sqrt((double)(a.g.pt - k + 1) / a.g.M));
+          }
+          var = np_mean10(ev, 10 - k);
+          int r = 0;
+          if (a.scale_f64) {
+            const double thr = (double)var * a.ts2;
+            for (int q = 0; q < 10; ++q) r += ((double)ev[q] > thr);
+          } else {
+            const RT thr = var * (RT)a.ts2;
+            for (int q = 0; q < 10; ++q) r += (ev[q] > thr);
+          }
+          rank = r;
+        } else {
+          // lowrank.py:108-112
+          var = np_mean10(ev, 10);
+          while ((double)var > prior && k < 10 - 2) {
+            ++k;
+            var = np_mean10(ev, 10 - k);
+          }
+          rank = k;
+        }
+        noise = (float)var;
+        ncomp = min(rank, nt);
+        for (int c = 0; c < ncomp; ++c) {
+          cidx[c] = n - 1 - c;
+          lam[c] = s.val[c];
+          wc[c] = 1.0;
+        }
+        if (a.EIG)
+          for (int q = 0; q < 10; ++q) a.EIG[(size_t)p * 10 + q] = (float)ev[q];
+        sh_i[0] = ncomp;
+        sh_i[1] = rank;
+        sh_d[0] = (double)noise;
+      }
+      wave_sync();
+      ncomp = sh_i[0];
+      rank = sh_i[1];
+      noise = (float)sh_d[0];
+    } else if (a.method == PDAMD_HARD) {
+      // lowrank.py:175-180: r = #{s > tau}; keep those components unchanged
+      const double tlo = a.tau > 0 ? a.tau * (1.0 - 1e-5) : a.tau;
+      int cand = n;
+      if (tlo > 0) {
+        const R x = (R)(tlo * tlo);
+        cand = n - sturm(s.d, s.e2, n, x, pivmin);
+      }
+      for (int j = lane; j < cand; j += kWave) s.tgt[j] = n - 1 - j;
+      wave_sync();
+      multisect(s, n, pivmin, glo, ghi, atol, cand);
+      if (lane == 0) {
+        int r = 0;
+        for (int j = 0; j < cand; ++j) {
+          const RT sv = (RT)sqrt(fmax((double)s.val[j], 0.0));
+          const bool keep = a.tau_f64 ? ((double)sv > a.tau) : (sv > (RT)a.tau);
+          if (!keep) break;
+          ++r;
+        }
+        rank = r;
+        ncomp = r;
+        for (int c = 0; c < r; ++c) {
+          cidx[c] = n - 1 - c;
+          lam[c] = s.val[c];
+          wc[c] = 1.0;
+        }
+        sh_i[0] = ncomp;
+        sh_i[1] = rank;
+      }
+      wave_sync();
+      ncomp = sh_i[0];
+      rank = sh_i[1];
+    } else {
+      // ---- optimal shrinkage, lowrank.py:387-407
+      double sigma;
+      if (a.var_map) {
+        sigma = prior;
+      } else {
+        const int nt = (n % 2) ? 1 : 2;
+        if (lane == 0) {
+          s.tgt[0] = (n - 1) / 2;
+          s.tgt[1] = n / 2;
+        }
+        wave_sync();
+        multisect(s, n, pivmin, glo, ghi, atol, nt);
+        RT med;
+        if (nt == 1) {
+          med = (RT)sqrt(fmax((double)s.val[0], 0.0));
+        } else {
+          const RT s0 = (RT)sqrt(fmax((double)s.val[0], 0.0));
+          const RT s1 = (RT)sqrt(fmax((double)s.val[1], 0.0));
+          med = (RT)(s0 + s1) / RT(2);
+        }
+        sigma = (double)med / sqrt((double)a.g.pt * a.mpm);
+      }
+      const double cs = sqrt((double)a.g.pt) * sigma;
+      const double sb = sqrt(a.beta);
+      // candidates above the upper bulk edge, and (Frobenius) below the lower one
+      int ctop = n, cbot = 0;
+      {
+        const double xt = cs * (1.0 + sb) * (1.0 - 1e-5);
+        if (xt > 0 && isfinite(xt)) ctop = n - sturm(s.d, s.e2, n, (R)(xt * xt), pivmin);
+        if (a.loss == PDAMD_LOSS_FRO && 1.0 - sb > 0) {
+          const double xb = cs * (1.0 - sb) * (1.0 + 1e-5);
+          if (isfinite(xb)) cbot = sturm(s.d, s.e2, n, (R)(xb * xb), pivmin);
+          if (!isfinite(xb)) cbot = n;
+        }
+        if (!isfinite(cs) || cs <= 0) {
+          ctop = n;
+          cbot = 0;
+        }
+        if (ctop + cbot > n) cbot = n - ctop;
+      }
+      for (int j = lane; j < ctop; j += kWave) s.tgt[j] = n - 1 - j;
+      for (int j = lane; j < cbot; j += kWave) s.tgt[ctop + j] = j;
+      wave_sync();
+      multisect(s, n, pivmin, glo, ghi, atol, ctop + cbot);
+      if (lane == 0) {
+        int nc = 0, last = -1;
+        for (int j = 0; j < ctop + cbot; ++j) {
+          const double lamj = (double)s.val[j];
+          const RT sv = (RT)sqrt(fmax(lamj, 0.0));
+          double sh = cs * shrink(a.loss, (double)sv / cs, a.beta);
+          if (isnan(sh)) sh = 0.0;
+          if (sh != 0.0) {
+            const int desc = j < ctop ? j : (n - 1 - s.tgt[j]);
+            last = max(last, desc);
+            cidx[nc] = s.tgt[j];
+            lam[nc] = s.val[j];
+            wc[nc] = sh / sqrt(fmax(lamj, 0.0));
+            ++nc;
+          }
+        }
+        ncomp = nc;
+        rank = last + 1;
+        sh_i[0] = ncomp;
+        sh_i[1] = rank;
+      }
+      wave_sync();
+      ncomp = sh_i[0];
+      rank = sh_i[1];
+    }
+    ncomp = min(ncomp, a.kmax);
+
+    // ---------------- eigenvectors of the kept components
+    const E* Hp = reinterpret_cast<const E*>(a.H) + (size_t)p * n * n;
+    const E* TAUp = reinterpret_cast<const E*>(a.TAU) + (size_t)p * n;
+    E* Yp = reinterpret_cast<E*>(a.Y) + (size_t)p * n * a.kmax;
+    R* Zp = reinterpret_cast<R*>(a.Z) + (size_t)p * n * a.kmax;
+    const R eps_piv = Lim<R>::ulp() * tnorm + pivmin;
+    const R clus = R(1e-3) * tnorm;
+    for (int c0 = 0; c0 < ncomp; c0 += KB) {
+      const int kb = min(KB, ncomp - c0);
+      // inverse iteration, one lane per component
+      if (lane < kb) {
+        const R lm = lam[c0 + lane];
+        uint32_t h = 0x9E3779B9u * (uint32_t)(c0 + lane + 1);
+        for (int i = 0; i < n; ++i) {
+          h ^= h << 13; h ^= h >> 17; h ^= h << 5;
+          zb[i * KB + lane] = R(0.5) + (R)(h & 0xFFFF) * R(1.0 / 65536.0);
+        }
+        for (int it = 0; it < 3; ++it) {
+          tri_solve(s.d, s.e, n, lm, u0 + lane, u1 + lane, u2 + lane, zb + lane, KB, eps_piv);
+          R mx = R(0);
+          for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(zb[i * KB + lane]));
+          const R inv = mx > R(0) ? R(1) / mx : R(1);
+          for (int i = 0; i < n; ++i) zb[i * KB + lane] *= inv;
+        }
+      }
+      wave_sync();
+      // Gram-Schmidt within clusters (this batch and earlier batches), twice
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int c = 0; c < kb; ++c) {
+          const int gc = c0 + c;
+          for (int q = gc - 1; q >= 0; --q) {
+            if (fabs(lam[q] - lam[gc]) > clus) continue;
+            R part = R(0);
+            if (q >= c0) {
+              for (int i = lane; i < n; i += kWave) part += zb[i * KB + (q - c0)] * zb[i * KB + c];
+            } else {
+              for (int i = lane; i < n; i += kWave) part += Zp[(size_t)i * a.kmax + q] * zb[i * KB + c];
+            }
+            const R dot = wave_sum(part);
+            if (q >= c0) {
+              for (int i = lane; i < n; i += kWave) zb[i * KB + c] -= dot * zb[i * KB + (q - c0)];
+            } else {
+              for (int i = lane; i < n; i += kWave) zb[i * KB + c] -= dot * Zp[(size_t)i * a.kmax + q];
+            }
+            wave_sync();
+          }
+          R part = R(0);
+          for (int i = lane; i < n; i += kWave) part += zb[i * KB + c] * zb[i * KB + c];
+          const R nrm = sqrt(wave_sum(part));
+          const R inv = nrm > R(0) ? R(1) / nrm : R(0);
+          for (int i = lane; i < n; i += kWave) zb[i * KB + c] *= inv;
+          wave_sync();
+        }
+        if (pass == 0 && lane < kb) {  // one more inverse-iteration step
+          tri_solve(s.d, s.e, n, lam[c0 + lane], u0 + lane, u1 + lane, u2 + lane, zb + lane, KB,
+                    eps_piv);
+          R mx = R(0);
+          for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(zb[i * KB + lane]));
+          const R inv = mx > R(0) ? R(1) / mx : R(1);
+          for (int i = 0; i < n; ++i) zb[i * KB + lane] *= inv;
+        }
+        wave_sync();
+      }
+      // keep z for clustering against later batches (read back through global
+      // memory by other lanes: full fence)
+      for (int q = lane; q < n * kb; q += kWave) {
+        const int i = q / kb, c = q % kb;
+        Zp[(size_t)i * a.kmax + c0 + c] = zb[i * KB + c];
+      }
+      if (c0 + kb < ncomp) __syncthreads();
+      // back-transformation y = H_0 ... H_{n-2} z
+      for (int q = lane; q < n * kb; q += kWave) {
+        const int i = q / kb, c = q % kb;
+        yb[i * KB + c] = E(zb[i * KB + c]);
+      }
+      wave_sync();
+      // reflectors are applied from k = n-2 down; the next one is prefetched
+      auto fetch = [&](int k, E& va, E& vb, E& tk) {
+        const int m = n - k - 1;
+        const E* vk = Hp + (size_t)k * n + k + 1;
+        tk = TAUp[k];
+        va = (lane < m) ? ((lane == 0) ? E(R(1)) : vk[lane]) : E(R(0));
+        vb = (lane + kWave < m) ? vk[lane + kWave] : E(R(0));
+      };
+      E va_n, vb_n, tk_n;
+      if (n >= 2) fetch(n - 2, va_n, vb_n, tk_n);
+      for (int k = n - 2; k >= 0; --k) {
+        const E tk = tk_n, va = va_n, vb = vb_n;
+        if (k > 0) fetch(k - 1, va_n, vb_n, tk_n);
+        if (re_(tk) == R(0) && im_(tk) == R(0)) continue;
+        const int m = n - k - 1;
+        for (int c = 0; c < kb; ++c) {
+          E part = E(R(0));
+          if (lane < m) part += conj_(va) * yb[(k + 1 + lane) * KB + c];
+          if (lane + kWave < m) part += conj_(vb) * yb[(k + 1 + lane + kWave) * KB + c];
+          const E sdot = tk * wave_sum(part);
+          if (lane < m) yb[(k + 1 + lane) * KB + c] -= va * sdot;
+          if (lane + kWave < m) yb[(k + 1 + lane + kWave) * KB + c] -= vb * sdot;
+        }
+        wave_sync();
+      }
+      for (int q = lane; q < n * kb; q += kWave) {
+        const int i = q / kb, c = q % kb;
+        Yp[(size_t)i * a.kmax + c0 + c] = yb[i * KB + c];
+      }
+      wave_sync();
+    }
+    // ---------------- per-patch scalars
+    if (lane == 0) {
+      a.NCOMP[p] = ncomp;
+      a.THETA[p] = 1.0 / (2.0 + (double)rank);
+      if (a.RANK) a.RANK[p] = rank;
+      if (a.NOISE) a.NOISE[p] = noise;
+    }
+    for (int c = lane; c < ncomp; c += kWave) a.WC[(size_t)p * a.kmax + c] = wc[c];
+    wave_sync();
+  }
+}
+
+template <class E>
+size_t trieig_lds_bytes(int n, int kb) {
+  using R = typename real_of<E>::type;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  size_t s = 3 * al(n * sizeof(R)) + 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int));
+  s += al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(R));
+  s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(double));
+  s += 4 * al((size_t)n * kb * sizeof(R)) + al((size_t)n * kb * sizeof(E));
+  return s;
+}
+
+// ===========================================================================
+// 4. Synthesis + scatter-accumulate recombination (base.py:243-256)
+// ===========================================================================
+
+struct SynthArgs {
+  Geom g;
+  const int32_t* idx;
+  int np, n, kmax;
+  int recomb;
+  int w3d;  // weights accumulated per voxel (one temporal patch) instead of per element
+  const void* Y;
+  const double* WC;
+  const int* NCOMP;
+  const double* THETA;
+  void* out;
+  float* w;  // w3d ? voxel weights : element weights
+};
+
+__device__ __forceinline__ void atomic_add_(float* p, float v) { atomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_(double* p, double v) { atomicAdd(p, v); }
+template <class R> __device__ __forceinline__ void atomic_add_(cpx<R>* p, cpx<R> v) {
+  R* q = reinterpret_cast<R*>(p);
+  atomicAdd(q, v.x);
+  atomicAdd(q + 1, v.y);
+}
+
+constexpr int kSynSC = 32;  // frames (M-side) or rows (T-side) per staged chunk
+constexpr int kSynKG = 16;  // components per group held in LDS
+constexpr int kSynNI = 16;  // output items per thread per chunk (128 x 32 / 256)
+
+template <class IN, class E, bool TSIDE>
+__global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, SynthArgs a) {
+  using R = typename real_of<E>::type;
+  constexpr int SC = kSynSC, KG = kSynKG;
+  const Geom& g = a.g;
+  const int n = a.n, tid = threadIdx.x;
+  const int M = g.M, T = g.pt;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  E* Ys = lds_carve<E>(cur, (size_t)n * KG);  // component group, Ys[i * KG + c]
+  R* wcs = lds_carve<R>(cur, KG);
+  const int ldw = TSIDE ? (T + 1) : (M + 1);
+  IN* Sraw = lds_carve<IN>(cur, (size_t)SC * ldw);
+  E* Sc = lds_carve<E>(cur, (size_t)SC * ldw);
+  IN* mean = lds_carve<IN>(cur, TSIDE ? T : SC);
+  E* B = lds_carve<E>(cur, (size_t)KG * SC);  // M-side: B[c][t]; T-side: C[r][c]
+  int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : M);
+
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    int64_t sp;
+    int t0;
+    patch_origin(g, a.idx[p], sp, t0);
+    const IN* X = vol + sp + t0;
+    IN* O = reinterpret_cast<IN*>(a.out) + sp + t0;
+    const int K = a.NCOMP[p];
+    const double theta = a.recomb == PDAMD_WEIGHTED ? a.THETA[p] : 1.0;
+    const E* Yp = reinterpret_cast<const E*>(a.Y) + (size_t)p * n * a.kmax;
+    const double* WCp = a.WC + (size_t)p * a.kmax;
+    const bool center = a.recomb == PDAMD_CENTER;
+    const int ctr_r = ((g.p0 / 2) * g.p1 + g.p1 / 2) * g.p2 + g.p2 / 2;
+    const int ctr_t = T / 2;
+    E acc[kSynNI];
+
+    if (!TSIDE) {
+      for (int r = tid; r < M; r += kNT) roff[r] = row_offset(g, r);
+      lds_barrier();
+      for (int c0 = 0; c0 < T; c0 += SC) {
+        const int tc = min(SC, T - c0);
+        if (center && (ctr_t < c0 || ctr_t >= c0 + tc)) continue;
+        {
+          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          IN buf[NL];
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            const int t = q % SC, r = q / SC;
+            buf[u] = (q < SC * M && t < tc) ? X[roff[r] + c0 + t] : IN(0);
+          }
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            if (q < SC * M) Sraw[(q % SC) * ldw + q / SC] = buf[u];
+          }
+        }
+        lds_barrier();
+        if (tid < tc)  // sequential over rows, as numpy's axis-0 mean
+          mean[tid] = div_n(seq_add(Sraw[tid * ldw], Sraw + tid * ldw + 1, M - 1, 1), M);
+        lds_barrier();
+        for (int q = tid; q < tc * M; q += kNT) {
+          const int t = q / M, r = q % M;
+          Sc[t * ldw + r] = cvt<E>(Sraw[t * ldw + r] - mean[t]);
+        }
+#pragma unroll
+        for (int j = 0; j < kSynNI; ++j) {
+          const int q = tid + j * kNT;
+          acc[j] = q < M * SC ? cvt<E>(mean[q % SC]) : E(R(0));
+        }
+        lds_barrier();
+        for (int k0 = 0; k0 < K; k0 += KG) {
+          const int kg = min(KG, K - k0);
+          for (int q = tid; q < n * kg; q += kNT) {
+            const int i = q / kg, c = q % kg;
+            Ys[i * KG + c] = Yp[(size_t)i * a.kmax + k0 + c];
+          }
+          for (int c = tid; c < kg; c += kNT) wcs[c] = (R)WCp[k0 + c];
+          lds_barrier();
+          // B[c][t] = w_c * sum_r conj(Y[r][c]) Xc[r][t]
+          for (int q = tid; q < kg * SC; q += kNT) {
+            const int c = q / SC, t = q % SC;
+            E b = E(R(0));
+            if (t < tc)
+              for (int r = 0; r < M; ++r) b += conj_(Ys[r * KG + c]) * Sc[t * ldw + r];
+            B[c * SC + t] = wcs[c] * b;
+          }
+          lds_barrier();
+#pragma unroll
+          for (int j = 0; j < kSynNI; ++j) {
+            const int q = tid + j * kNT;
+            if (q < M * SC) {
+              const int r = q / SC, t = q % SC;
+              E v = acc[j];
+              for (int c = 0; c < kg; ++c) v += Ys[r * KG + c] * B[c * SC + t];
+              acc[j] = v;
+            }
+          }
+          lds_barrier();
+        }
+#pragma unroll
+        for (int j = 0; j < kSynNI; ++j) {
+          const int q = tid + j * kNT;
+          if (q >= M * SC) continue;
+          const int r = q / SC, t = q % SC;
+          if (t >= tc) continue;
+          IN* dst = O + roff[r] + c0 + t;
+          if (center) {
+            if (r == ctr_r && c0 + t == ctr_t) *dst = cvt<IN>(acc[j]);
+          } else {
+            atomic_add_(dst, cvt<IN>((R)theta * acc[j]));
+          }
+        }
+        lds_barrier();
+      }
+    } else {
+      // pass 1: per-frame means over all rows (sequential order)
+      IN sm = IN(0);
+      for (int r0 = 0; r0 < M; r0 += SC) {
+        const int rc = min(SC, M - r0);
+        for (int r = tid; r < rc; r += kNT) roff[r] = row_offset(g, r0 + r);
+        lds_barrier();
+        {
+          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          IN buf[NL];
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            buf[u] = q < rc * T ? X[roff[q / T] + q % T] : IN(0);
+          }
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            if (q < rc * T) Sraw[(q / T) * ldw + q % T] = buf[u];
+          }
+        }
+        lds_barrier();
+        if (tid < T) {
+          if (r0 == 0) sm = seq_add(Sraw[tid], Sraw + ldw + tid, rc - 1, ldw);
+          else sm = seq_add(sm, Sraw + tid, rc, ldw);
+        }
+        lds_barrier();
+      }
+      for (int t = tid; t < T; t += kNT) mean[t] = div_n(sm, M);
+      lds_barrier();
+      for (int r0 = 0; r0 < M; r0 += SC) {
+        const int rc = min(SC, M - r0);
+        if (center && (ctr_r < r0 || ctr_r >= r0 + rc)) continue;
+        for (int r = tid; r < rc; r += kNT) roff[r] = row_offset(g, r0 + r);
+        lds_barrier();
+        {
+          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          IN buf[NL];
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            buf[u] = q < rc * T ? X[roff[q / T] + q % T] : IN(0);
+          }
+#pragma unroll
+          for (int u = 0; u < NL; ++u) {
+            const int q = tid + u * kNT;
+            if (q < rc * T) Sc[(q / T) * ldw + q % T] = cvt<E>(buf[u] - mean[q % T]);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kSynNI; ++j) {
+          const int q = tid + j * kNT;
+          acc[j] = q < rc * T ? cvt<E>(mean[q % T]) : E(R(0));
+        }
+        lds_barrier();
+        for (int k0 = 0; k0 < K; k0 += KG) {
+          const int kg = min(KG, K - k0);
+          for (int q = tid; q < n * kg; q += kNT) {
+            const int i = q / kg, c = q % kg;
+            Ys[i * KG + c] = Yp[(size_t)i * a.kmax + k0 + c];
+          }
+          for (int c = tid; c < kg; c += kNT) wcs[c] = (R)WCp[k0 + c];
+          lds_barrier();
+          // C[r][c] = w_c * sum_t Xc[r][t] Y[t][c]
+          for (int q = tid; q < rc * kg; q += kNT) {
+            const int r = q / kg, c = q % kg;
+            E b = E(R(0));
+            for (int t = 0; t < T; ++t) b += Sc[r * ldw + t] * Ys[t * KG + c];
+            B[r * KG + c] = wcs[c] * b;
+          }
+          lds_barrier();
+#pragma unroll
+          for (int j = 0; j < kSynNI; ++j) {
+            const int q = tid + j * kNT;
+            if (q < rc * T) {
+              const int r = q / T, t = q % T;
+              E v = acc[j];
+              for (int c = 0; c < kg; ++c) v += B[r * KG + c] * conj_(Ys[t * KG + c]);
+              acc[j] = v;
+            }
+          }
+          lds_barrier();
+        }
+#pragma unroll
+        for (int j = 0; j < kSynNI; ++j) {
+          const int q = tid + j * kNT;
+          if (q >= rc * T) continue;
+          const int r = q / T, t = q % T;
+          IN* dst = O + roff[r] + t;
+          if (center) {
+            if (r0 + r == ctr_r && t == ctr_t) *dst = cvt<IN>(acc[j]);
+          } else {
+            atomic_add_(dst, cvt<IN>((R)theta * acc[j]));
+          }
+        }
+        lds_barrier();
+      }
+    }
+    // weights
+    if (!center) {
+      if (a.w3d) {
+        for (int r = tid; r < M; r += kNT)
+          atomicAdd(a.w + (sp + row_offset(g, r)) / g.T, (float)theta);
+      } else {
+        for (int q = tid; q < M * T; q += kNT) {
+          const int r = q / T, t = q % T;
+          atomicAdd(a.w + sp + row_offset(g, r) + t0 + t, (float)theta);
+        }
+      }
+    }
+    lds_barrier();
+  }
+}
+
+template <class IN, class E, bool TSIDE>
+size_t synth_lds_bytes(int n, int kmax, int M, int T) {
+  using R = typename real_of<E>::type;
+  constexpr int SC = kSynSC, KG = kSynKG;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  const int ldw = TSIDE ? (T + 1) : (M + 1);
+  size_t s = al((size_t)n * KG * sizeof(E)) + al(KG * sizeof(R));
+  s += al((size_t)SC * ldw * sizeof(IN)) + al((size_t)SC * ldw * sizeof(E));
+  s += al((TSIDE ? T : SC) * sizeof(IN)) + al((size_t)KG * SC * sizeof(E));
+  s += al((TSIDE ? SC : M) * sizeof(int64_t));
+  return s;
+}
+
+// ===========================================================================
+// 5. Normalisation and mask (base.py:262-270)
+// ===========================================================================
+
+__device__ __forceinline__ float div_w(float a, float w) { return a / w; }
+__device__ __forceinline__ double div_w(double a, float w) { return a / (double)w; }
+template <class R> __device__ __forceinline__ cpx<R> div_w(cpx<R> a, float w) {
+  return {a.x / (R)w, a.y / (R)w};
+}
+
+template <class IN>
+__global__ void finalize_kernel(IN* __restrict__ out, float* __restrict__ w4,
+                                const float* __restrict__ w3, const uint8_t* __restrict__ mask,
+                                int mask_kind, int recomb, int64_t nvox, int64_t T) {
+  const int64_t total = nvox * T;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = q / T;
+    float wv;
+    if (w3) {
+      wv = w3[v];
+      if (recomb != PDAMD_CENTER) w4[q] = wv;
+    } else {
+      wv = w4[q];
+    }
+    IN o = out[q];
+    if (recomb != PDAMD_CENTER) o = div_w(o, wv);
+    bool keep = true;
+    if (mask_kind == PDAMD_MASK_SPATIAL) keep = mask[v] != 0;
+    else if (mask_kind == PDAMD_MASK_FULL) keep = mask[q] != 0;
+    if (!keep) o = IN(0);
+    out[q] = o;
+  }
+}
+
+// ===========================================================================
+// Host side: error handling, dispatch, workspace, driver
+// ===========================================================================
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) return fail(PDAMD_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+// Optional per-stage timing with HIP events on the work stream.
+struct StageTimer {
+  bool on = false;
+  hipStream_t st = nullptr;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  int cur = -1;
+  hipEvent_t cur_a = nullptr;
+  void begin(int stage) {
+    if (!on) return;
+    cur = stage;
+    hipEventCreate(&cur_a);
+    hipEventRecord(cur_a, st);
+  }
+  void end() {
+    if (!on || cur < 0) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, st);
+    recs.push_back({cur, cur_a, b});
+    cur = -1;
+  }
+  int collect(float* ms, int64_t* launches) {
+    if (!on) return 0;
+    hipError_t e = hipStreamSynchronize(st);
+    for (auto& r : recs) {
+      float t = 0.f;
+      if (e == hipSuccess) hipEventElapsedTime(&t, r.a, r.b);
+      if (ms) ms[r.stage] += t;
+      if (launches) launches[r.stage] += 1;
+      hipEventDestroy(r.a);
+      hipEventDestroy(r.b);
+    }
+    recs.clear();
+    return e == hipSuccess ? 0 : -1;
+  }
+};
+
+struct Plan {
+  Geom g;
+  int64_t npatch, nvox;
+  int n, kmax, kb, tile;
+  bool tside;
+  int etype;  // 0 float, 1 double, 2 cpx<float>, 3 cpx<double>
+  int rule_f64;
+  int64_t chunk;
+  size_t ebytes, rbytes;
+  // workspace offsets
+  size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
+      off_WC, off_NC, off_TH, off_w3, total;
+};
+
+int make_plan(const pdamd_problem* P, Plan* pl) {
+  if (!P) return fail(PDAMD_EINVAL, "null problem");
+  if (P->dtype < 0 || P->dtype > 3) return fail(PDAMD_EINVAL, "bad dtype %d", P->dtype);
+  for (int i = 0; i < 4; ++i) {
+    if (P->shape[i] < 1 || P->patch[i] < 1 || P->patch[i] > P->shape[i] || P->step[i] < 0)
+      return fail(PDAMD_EINVAL, "bad shape/patch/step in dim %d", i);
+  }
+  if (P->method < 0 || P->method > 3) return fail(PDAMD_EINVAL, "bad method %d", P->method);
+  if (P->recombination < 0 || P->recombination > 2)
+    return fail(PDAMD_EINVAL, "recombination must be one of 'weighted', 'average', 'center'");
+  Geom& g = pl->g;
+  const int64_t T = P->shape[3];
+  g.T = T;
+  g.vs2 = T;
+  g.vs1 = P->shape[2] * T;
+  g.vs0 = P->shape[1] * P->shape[2] * T;
+  int* gg[4] = {&g.g0, &g.g1, &g.g2, &g.gt};
+  int* ss[4] = {&g.s0, &g.s1, &g.s2, &g.st};
+  int* pp[4] = {&g.p0, &g.p1, &g.p2, &g.pt};
+  pl->npatch = 1;
+  for (int i = 0; i < 4; ++i) {
+    const int64_t S = P->shape[i], p = P->patch[i], st = P->step[i];
+    const int64_t gd = (p < S) ? (st > 0 ? (S - p) / st + 1 : 1) : 1;
+    *gg[i] = (int)gd;
+    *ss[i] = (int)((p < S) ? st : 0);
+    *pp[i] = (int)p;
+    pl->npatch *= gd;
+  }
+  if (pl->npatch >= (int64_t)1 << 31) return fail(PDAMD_EUNSUPPORTED, "too many patches");
+  g.M = g.p0 * g.p1 * g.p2;
+  pl->nvox = P->shape[0] * P->shape[1] * P->shape[2];
+  const int M = g.M, pt = g.pt;
+  pl->tside = pt < M;
+  pl->n = pl->tside ? pt : M;
+  const int n = pl->n;
+  if (n > 128)
+    return fail(PDAMD_EUNSUPPORTED,
+                "min(patch voxels, patch frames) = %d exceeds 128 (supported maximum)", n);
+  if (n < 2) return fail(PDAMD_EUNSUPPORTED, "patch too small (min side %d)", n);
+  if ((P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) && pt < 10)
+    return fail(PDAMD_EINVAL, "the eigen rules need at least 10 frames per patch");
+  pl->tile = n <= 32 ? 2 : (n <= 64 ? 4 : 8);
+  const bool cplx = P->dtype == PDAMD_C64 || P->dtype == PDAMD_C128;
+  const bool in64 = P->dtype == PDAMD_F64 || P->dtype == PDAMD_C128;
+  pl->rule_f64 = in64 ? 1 : 0;
+  bool e64;
+  if (P->precision == PDAMD_PREC_FP32) e64 = in64;  // never below the input precision
+  else if (P->precision == PDAMD_PREC_FP64) e64 = true;
+  else e64 = in64 || (P->method == PDAMD_HARD || P->method == PDAMD_OPTIMAL);
+  if (cplx && n > 64 && e64 && !in64) {
+    // complex double Gram of n > 64 does not fit LDS with this kernel
+  }
+  pl->etype = (cplx ? 2 : 0) + (e64 ? 1 : 0);
+  pl->rbytes = e64 ? 8 : 4;
+  pl->ebytes = pl->rbytes * (cplx ? 2 : 1);
+  pl->kmax = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? std::min(10, n) : n;
+  pl->kb = std::min(16, pl->kmax);
+  // chunk: keep the Householder block under ~1 GiB
+  const size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
+                     3 * n * pl->ebytes + pl->kmax * 8 + 16;
+  int64_t chunk = (int64_t)((1ull << 30) / per);
+  chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
+  pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
+  // workspace layout
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    size_t r = o;
+    o += ((b + 255) / 256) * 256;
+    return r;
+  };
+  const int64_t nsel_blocks = (pl->npatch + kSelPerBlock - 1) / kSelPerBlock;
+  pl->off_cnt = take((size_t)pl->nvox * g.gt * sizeof(int));
+  pl->off_flags = take((size_t)pl->npatch);
+  pl->off_btot = take((size_t)nsel_blocks * sizeof(int));
+  pl->off_idx = take((size_t)pl->npatch * sizeof(int32_t));
+  pl->off_count = take(sizeof(int64_t));
+  const size_t C = (size_t)pl->chunk;
+  pl->off_H = take(C * n * n * pl->ebytes);
+  pl->off_DE = take(C * 2 * n * pl->rbytes);
+  pl->off_TAU = take(C * n * pl->ebytes);
+  pl->off_Y = take(C * n * pl->kmax * pl->ebytes);
+  pl->off_Z = take(C * n * pl->kmax * pl->rbytes);
+  pl->off_WC = take(C * pl->kmax * sizeof(double));
+  pl->off_NC = take(C * sizeof(int));
+  pl->off_TH = take(C * sizeof(double));
+  pl->off_w3 = take((size_t)pl->nvox * sizeof(float));
+  pl->total = o;
+  return PDAMD_OK;
+}
+
+int run_select(const pdamd_problem* P, const Plan& pl, char* ws, int32_t* idx_out,
+               int64_t* count_host, hipStream_t st, StageTimer* tm = nullptr) {
+  int* cnt = reinterpret_cast<int*>(ws + pl.off_cnt);
+  uint8_t* flags = reinterpret_cast<uint8_t*>(ws + pl.off_flags);
+  int* btot = reinterpret_cast<int*>(ws + pl.off_btot);
+  int64_t* dcount = reinterpret_cast<int64_t*>(ws + pl.off_count);
+  const int kind = P->mask_kind;
+  if (kind != PDAMD_MASK_NONE && !P->mask_dev) return fail(PDAMD_EINVAL, "mask pointer missing");
+  if (tm) tm->begin(PDAMD_STAGE_SELECT);
+  if (kind != PDAMD_MASK_NONE) {
+    const int64_t tot = pl.nvox * pl.g.gt;
+    const int nb = (int)((tot + 255) / 256);
+    mask_window_kernel<<<nb, 256, 0, st>>>(P->mask_dev, kind, pl.nvox, pl.g.T, pl.g.gt, pl.g.st,
+                                           pl.g.pt, cnt);
+  }
+  const int nblk = (int)((pl.npatch + kSelPerBlock - 1) / kSelPerBlock);
+  const double psize = (double)pl.g.M * (double)pl.g.pt;
+  select_kernel<<<nblk, kNT, 0, st>>>(cnt, pl.g, pl.npatch, kind, P->mask_threshold, psize,
+                                      flags, btot);
+  scan_kernel<<<1, 1024, 0, st>>>(btot, nblk, dcount);
+  compact_kernel<<<nblk, kNT, 0, st>>>(flags, pl.npatch, btot, idx_out);
+  if (tm) tm->end();
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(count_host, dcount, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return PDAMD_OK;
+}
+
+int grid_for(int64_t np, int per_cu) {
+  static int n_cu = 0;
+  if (!n_cu) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (n_cu <= 0) n_cu = 256;
+  }
+  return (int)std::max<int64_t>(1, std::min<int64_t>(np, (int64_t)n_cu * per_cu));
+}
+
+template <class IN, class E, bool TSIDE, int TILE>
+int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n);
+  if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
+  auto kern = gram_tridiag_kernel<IN, E, TSIDE, TILE>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max(1, (int)((160 * 1024) / lds));
+  kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class IN, class E, bool TSIDE>
+int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  switch (pl.tile) {
+    case 2: return launch_gram<IN, E, TSIDE, 2>(pl, vol, ga, st);
+    case 4: return launch_gram<IN, E, TSIDE, 4>(pl, vol, ga, st);
+    default: return launch_gram<IN, E, TSIDE, 8>(pl, vol, ga, st);
+  }
+}
+
+template <class E, class RT>
+int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
+  const size_t lds = trieig_lds_bytes<E>(pl.n, pl.kb);
+  if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "eigen LDS %zu bytes exceeds 160 KiB", lds);
+  auto kern = trieig_kernel<E, RT>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max(1, std::min(16, (int)((160 * 1024) / lds)));
+  kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class IN, class E, bool TSIDE>
+int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) {
+  const size_t lds = synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
+  if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "synth LDS %zu bytes exceeds 160 KiB", lds);
+  auto kern = synth_kernel<IN, E, TSIDE>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+  kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class IN, class E, class RT>
+int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
+               StageTimer& tm) {
+  const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
+  float* w3 = reinterpret_cast<float*>(ws + pl.off_w3);
+  const bool w3d = pl.g.gt == 1;
+  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
+    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+    GramArgs ga;
+    ga.g = pl.g;
+    ga.idx = idx_all + c0;
+    ga.np = np;
+    ga.n = pl.n;
+    ga.scale = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? 1 : 0;
+    ga.H = ws + pl.off_H;
+    ga.DE = ws + pl.off_DE;
+    ga.TAU = ws + pl.off_TAU;
+    tm.begin(PDAMD_STAGE_GRAM);
+    int rc = pl.tside ? launch_gram_tile<IN, E, true>(pl, P->input_dev, ga, st)
+                      : launch_gram_tile<IN, E, false>(pl, P->input_dev, ga, st);
+    tm.end();
+    if (rc) return rc;
+    EigArgs ea;
+    ea.g = pl.g;
+    ea.idx = ga.idx;
+    ea.np = np;
+    ea.n = pl.n;
+    ea.kmax = pl.kmax;
+    ea.kb = pl.kb;
+    ea.method = P->method;
+    ea.loss = P->loss;
+    ea.rule_f64 = pl.rule_f64;
+    ea.scale_f64 = P->scale_f64;
+    ea.tau_f64 = P->tau_f64;
+    ea.ts2 = P->threshold_scale2;
+    ea.tau = P->tau;
+    ea.mpm = P->mp_median;
+    ea.beta = (double)pl.g.pt / (double)pl.g.M;
+    ea.var_map = P->var_map_dev;
+    ea.H = ga.H;
+    ea.DE = ga.DE;
+    ea.TAU = ga.TAU;
+    ea.Y = ws + pl.off_Y;
+    ea.Z = ws + pl.off_Z;
+    ea.WC = reinterpret_cast<double*>(ws + pl.off_WC);
+    ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_NC);
+    ea.THETA = reinterpret_cast<double*>(ws + pl.off_TH);
+    ea.RANK = P->rank_dev ? P->rank_dev + c0 : nullptr;
+    ea.NOISE = P->noise_dev ? P->noise_dev + c0 : nullptr;
+    ea.EIG = P->eig_dev ? P->eig_dev + c0 * 10 : nullptr;
+    tm.begin(PDAMD_STAGE_EIG);
+    rc = launch_eig<E, RT>(pl, ea, st);
+    tm.end();
+    if (rc) return rc;
+    SynthArgs sa;
+    sa.g = pl.g;
+    sa.idx = ga.idx;
+    sa.np = np;
+    sa.n = pl.n;
+    sa.kmax = pl.kmax;
+    sa.recomb = P->recombination;
+    sa.w3d = w3d ? 1 : 0;
+    sa.Y = ea.Y;
+    sa.WC = ea.WC;
+    sa.NCOMP = ea.NCOMP;
+    sa.THETA = ea.THETA;
+    sa.out = P->output_dev;
+    sa.w = w3d ? w3 : P->weights_dev;
+    tm.begin(PDAMD_STAGE_SYNTH);
+    rc = pl.tside ? launch_synth<IN, E, true>(pl, P->input_dev, sa, st)
+                  : launch_synth<IN, E, false>(pl, P->input_dev, sa, st);
+    tm.end();
+    if (rc) return rc;
+  }
+  const int64_t total = pl.nvox * pl.g.T;
+  const int nb = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  tm.begin(PDAMD_STAGE_FINALIZE);
+  finalize_kernel<IN><<<nb, 256, 0, st>>>(reinterpret_cast<IN*>(P->output_dev), P->weights_dev,
+                                          w3d ? w3 : nullptr, P->mask_dev, P->mask_kind,
+                                          P->recombination, pl.nvox, pl.g.T);
+  tm.end();
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pdamd_last_error(void) { return g_err.c_str(); }
+
+const char* pdamd_version(void) { return "patch_denoise_amd 0.1 (gfx950)"; }
+
+int pdamd_workspace_size(const pdamd_problem* prob, size_t* bytes) {
+  Plan pl;
+  const int rc = make_plan(prob, &pl);
+  if (rc) return rc;
+  *bytes = pl.total;
+  return PDAMD_OK;
+}
+
+int pdamd_select_patches(const pdamd_problem* prob, int32_t* index_dev, int64_t* count,
+                         void* workspace_dev, size_t workspace_bytes, void* stream) {
+  Plan pl;
+  int rc = make_plan(prob, &pl);
+  if (rc) return rc;
+  if (workspace_bytes < pl.total) return fail(PDAMD_EWORKSPACE, "workspace too small");
+  return run_select(prob, pl, reinterpret_cast<char*>(workspace_dev), index_dev, count,
+                    reinterpret_cast<hipStream_t>(stream));
+}
+
+int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes, void* stream) {
+  Plan pl;
+  int rc = make_plan(P, &pl);
+  if (rc) return rc;
+  if (workspace_bytes < pl.total)
+    return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, pl.total);
+  if (!P->input_dev || !P->output_dev || !P->weights_dev)
+    return fail(PDAMD_EINVAL, "null data pointer");
+  if ((P->method == PDAMD_HYBRID) && !P->var_map_dev)
+    return fail(PDAMD_EINVAL, "hybrid rule needs a variance map");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* ws = reinterpret_cast<char*>(workspace_dev);
+  const size_t esz = (P->dtype == PDAMD_F32) ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
+  HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
+  HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
+  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0, (size_t)pl.nvox * sizeof(float), st));
+  int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
+  int64_t count = 0;
+  StageTimer tm;
+  tm.on = P->stage_ms != nullptr || P->stage_launches != nullptr;
+  tm.st = st;
+  rc = run_select(P, pl, ws, idx, &count, st, &tm);
+  if (rc) return rc;
+  P->selected = count;
+  if (P->patch_index_dev && count > 0)
+    HIP_TRY(hipMemcpyAsync(P->patch_index_dev, idx, count * sizeof(int32_t),
+                           hipMemcpyDeviceToDevice, st));
+  switch (P->dtype * 4 + pl.etype) {
+    case PDAMD_F32 * 4 + 0: rc = run_chunks<float, float, float>(P, pl, ws, count, st, tm); break;
+    case PDAMD_F32 * 4 + 1: rc = run_chunks<float, double, float>(P, pl, ws, count, st, tm); break;
+    case PDAMD_F64 * 4 + 1: rc = run_chunks<double, double, double>(P, pl, ws, count, st, tm); break;
+    case PDAMD_C64 * 4 + 2:
+      rc = run_chunks<cpx<float>, cpx<float>, float>(P, pl, ws, count, st, tm);
+      break;
+    case PDAMD_C64 * 4 + 3:
+      rc = run_chunks<cpx<float>, cpx<double>, float>(P, pl, ws, count, st, tm);
+      break;
+    case PDAMD_C128 * 4 + 3:
+      rc = run_chunks<cpx<double>, cpx<double>, double>(P, pl, ws, count, st, tm);
+      break;
+    default:
+      return fail(PDAMD_EUNSUPPORTED, "dtype/precision combination not built");
+  }
+  if (rc) return rc;
+  if (tm.collect(P->stage_ms, P->stage_launches))
+    return fail(PDAMD_EHIP, "stream failed while collecting stage timings");
+  return PDAMD_OK;
+}
+
+}  // extern "C"
