@@ -115,6 +115,39 @@ template <class R> __device__ __forceinline__ cpx<R> wave_sum(cpx<R> v) {
   return v;
 }
 
+// Wavefront sum with DPP row operations plus 4 readlanes: every lane gets the
+// same, bit-identical result (so waves that compute it redundantly agree).
+// Must be called with all 64 lanes active.
+template <int CTRL> __device__ __forceinline__ float dpp_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
+                                                               CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_f(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float readlane_(float x, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+}
+__device__ __forceinline__ double readlane_(double x, int lane) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <class R> __device__ __forceinline__ R dpp_sum(R v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  v += dpp_f<0x140>(v);  // row_mirror
+  return ((readlane_(v, 0) + readlane_(v, 16)) + readlane_(v, 32)) + readlane_(v, 48);
+}
+template <class R> __device__ __forceinline__ cpx<R> dpp_sum(cpx<R> v) {
+  return {dpp_sum(v.x), dpp_sum(v.y)};
+}
+
 // Sum over a block of NT threads; `red` is LDS scratch of NT/64 elements.
 // Every thread returns the total.  Contains two barriers.
 template <int NT, class T>

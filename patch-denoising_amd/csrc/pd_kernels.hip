@@ -154,129 +154,157 @@ __global__ __launch_bounds__(kNT) void compact_kernel(const uint8_t* __restrict_
 // 2. Gram + Householder tridiagonalisation (utils.py:102-137 / :58-78)
 // ===========================================================================
 
+// a - x*y - z*t as explicit fused multiply-adds.
+__device__ __forceinline__ float rank2_update(float a, float x, float y, float z, float t) {
+  return fmaf(-z, t, fmaf(-x, y, a));
+}
+__device__ __forceinline__ double rank2_update(double a, double x, double y, double z, double t) {
+  return fma(-z, t, fma(-x, y, a));
+}
+template <class R>
+__device__ __forceinline__ cpx<R> rank2_update(cpx<R> a, cpx<R> x, cpx<R> y, cpx<R> z, cpx<R> t) {
+  R re = fma(-x.x, y.x, a.x);
+  re = fma(x.y, y.y, re);
+  re = fma(-z.x, t.x, re);
+  re = fma(z.y, t.y, re);
+  R im = fma(-x.x, y.y, a.y);
+  im = fma(-x.y, y.x, im);
+  im = fma(-z.x, t.y, im);
+  im = fma(-z.y, t.x, im);
+  return {re, im};
+}
+
 // Householder tridiagonalisation of an n x n Hermitian matrix (n <= 128)
 // held in registers.  Thread (w = tid / 64, l = tid % 64) owns the entries
 // A[r][c] for rows r = w + 4 jj (jj < RJ) and columns c in {l, l + 64}.  The
 // reduction follows LAPACK ?hetd2 (lower): T = Q^H A Q with
-// Q = H_0 H_1 ... H_{n-2}, H_k = I - tau_k v_k v_k^H.  Per step only the
-// reflector and the vectors p / w travel through LDS (X, V, W, P), indexed by
-// absolute row / column with zeros at indices <= k, so the rank-2 update
-// needs no masking; rows <= k are skipped with wave-uniform branches.
-// Outputs: dd / ee (real tridiagonal), tt (tau_k), and v_k written to row k
-// of Hrow in LDS (row stride n, columns k+1 .. n-1, v_k[0] = 1).
+// Q = H_0 H_1 ... H_{n-2}, H_k = I - tau_k v_k v_k^H.
+//
+// Per step k, with two workgroup barriers:
+//   publish  the wave owning row k wrote conj(row k) = column k into X (and
+//            the diagonal into dd) at the end of step k-1's update;
+//   (b)      every wave forms the reflector from X (DPP sums give the same
+//            bits in every wave) and keeps v in registers for its own columns
+//            and, gathered by row slot, in its private LDS copy Vr;
+//   (c)      partial products p_c = sum over own rows of conj(A[r][c]) v_r -> P;
+//   (d)      every wave: p = tau sum_w P, K = -tau/2 p^H v, w = p + K v (own
+//            columns in registers, own rows in Wr);
+//   (e)      A -= v w^H + w v^H on own entries; the owner of row k+1 publishes
+//            it for the next step.
+// v and w are zero at indices <= k and >= n, so no entry needs masking; whole
+// quarters of inactive rows are skipped with wave-uniform branches.
+// Outputs: dd / ee (real tridiagonal), tt (tau_k), and v_k in row k of Hrow
+// (LDS, row stride n, columns k+1 .. n-1, v_k[0] = 1).
 template <class E, int RJ>
 __device__ void tridiag_reg(E (&a)[RJ][2], int n, typename real_of<E>::type* dd,
-                            typename real_of<E>::type* ee, E* tt, E* X, E* V, E* W, E* P,
+                            typename real_of<E>::type* ee, E* tt, E* X, E* Vall, E* Wall, E* P,
                             E* Hrow) {
   using R = typename real_of<E>::type;
   constexpr bool TWO = RJ * 4 > 64;  // second column set exists
-  const int tid = threadIdx.x, l = tid & (kWave - 1), w = tid / kWave;
+  constexpr int QJ = RJ / 4;
+  const int tid = threadIdx.x, l = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);  // wave-uniform (SGPR)
+  const int i0 = l, i1 = l + kWave;  // absolute columns held by this lane
   const E zero = E(R(0));
+  E* Vr = Vall + w * 32;  // v at this wave's rows, by row slot jj
+  E* Wr = Wall + w * 32;
+  // publish row k of the current matrix (conjugated) as column k
+  auto publish = [&](int jj, int k) {
+    const E c0 = a[jj][0], c1 = a[jj][1];
+    if (i0 > k && i0 < n) X[i0] = conj_(c0);
+    if (TWO && i1 > k && i1 < n) X[i1] = conj_(c1);
+    if (i0 == k) dd[k] = re_(c0);
+    if (TWO && i1 == k) dd[k] = re_(c1);
+  };
+  if (w == 0) publish(0, 0);
   for (int k = 0; k < n - 1; ++k) {
-    // (a) owners of column k publish A[r][k] (r > k) and the diagonal A[k][k]
-    if (l == (k & (kWave - 1))) {
-      const bool hi = k >= kWave;
-#pragma unroll
-      for (int jj = 0; jj < RJ; ++jj) {
-        const int r = w + 4 * jj;
-        if (r >= k && r < n) {
-          const E v = hi ? a[jj][TWO ? 1 : 0] : a[jj][0];
-          if (r == k) dd[k] = re_(v);
-          else X[r] = v;
-        }
-      }
-    }
     lds_barrier();
-    // (b) wave 0: reflector from x = X[k+1 .. n-1]
+    // (b) every wave: reflector from x = X[k+1 .. n-1]
+    const E x0 = (i0 > k && i0 < n) ? X[i0] : zero;
+    const E x1 = (i1 > k && i1 < n) ? X[i1] : zero;
+    const R xn2 = dpp_sum((i0 > k + 1 ? abs2_(x0) : R(0)) + (i1 > k + 1 ? abs2_(x1) : R(0)));
+    const E alpha = X[k + 1];
+    const R ar = re_(alpha), ai = im_(alpha);
+    const bool refl = (xn2 > R(0)) || (ai != R(0));
+    E v0 = zero, v1 = zero, tk = zero;
+    R beta = ar;
+    if (refl) {
+      beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
+      const R ibeta = R(1) / beta;
+      tk = mk<E>((beta - ar) * ibeta, -ai * ibeta);
+      const E scal = recip_(alpha - E(beta));
+      v0 = (i0 > k && i0 < n) ? ((i0 == k + 1) ? E(R(1)) : x0 * scal) : zero;
+      v1 = (i1 > k && i1 < n) ? ((i1 == k + 1) ? E(R(1)) : x1 * scal) : zero;
+    }
+    if ((i0 & 3) == w) Vr[i0 >> 2] = v0;
+    if ((i1 & 3) == w) Vr[i1 >> 2] = v1;
     if (w == 0) {
-      const int r0 = k + 1 + l, r1 = k + 1 + l + kWave;
-      const E x0 = r0 < n ? X[r0] : zero;
-      const E x1 = r1 < n ? X[r1] : zero;
-      const R xn2 = wave_sum((l > 0 ? abs2_(x0) : R(0)) + abs2_(x1));
-      const E alpha = X[k + 1];
-      const R ar = re_(alpha), ai = im_(alpha);
-      const bool refl = (xn2 > R(0)) || (ai != R(0));
-      R beta = ar;
-      E tk = zero, scal = zero;
-      if (refl) {
-        beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
-        tk = mk<E>((beta - ar) / beta, -ai / beta);
-        scal = recip_(alpha - E(beta));
-      }
-      // V over absolute indices: 0 for r <= k, 1 at k + 1, x_r * scal above
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = l + h * kWave;
-        E v = zero;
-        if (refl && r > k && r < n) v = (r == k + 1) ? E(R(1)) : X[r] * scal;
-        V[r] = v;
-        if (r > k && r < n) Hrow[(size_t)k * n + r] = v;
-      }
+      if (i0 > k && i0 < n) Hrow[(size_t)k * n + i0] = v0;
+      if (i1 > k && i1 < n) Hrow[(size_t)k * n + i1] = v1;
       if (l == 0) {
         tt[k] = tk;
         ee[k] = beta;
       }
     }
-    lds_barrier();
-    const E tk = tt[k];
-    if (re_(tk) == R(0) && im_(tk) == R(0)) continue;  // H_k = I
-    // (c) partial p_c = sum over my rows r of conj(A[r][c]) v_r.  V is zero at
-    //     rows <= k and >= n, so only whole quarters of inactive rows are skipped.
-    E p0 = zero, p1 = zero;
-    constexpr int QJ = RJ / 4;
+    if (!refl) {
+      // H_k = I (uniform: every wave computed the same xn2, alpha).  Row k+1
+      // is unchanged; publish it after every wave has read X.
+      lds_barrier();
+      if (w == ((k + 1) & 3)) {
+#pragma unroll
+        for (int jj = 0; jj < RJ; ++jj)
+          if (w + 4 * jj == k + 1) publish(jj, k + 1);
+      }
+      continue;
+    }
+    wave_sync();
+    // (c) partial p_c = sum over my rows r of conj(A[r][c]) v_r
+    E p0a = zero, p0b = zero, p1a = zero, p1b = zero;
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) {
       if (w + 4 * (qq * QJ + QJ - 1) > k) {
 #pragma unroll
-        for (int u = 0; u < QJ; ++u) {
+        for (int u = 0; u < QJ; u += 2) {
           const int jj = qq * QJ + u;
-          const E vr = V[w + 4 * jj];
-          p0 += conj_(a[jj][0]) * vr;
-          if (TWO) p1 += conj_(a[jj][1]) * vr;
+          const E va = Vr[jj], vb = Vr[jj + 1];
+          p0a += conj_(a[jj][0]) * va;
+          p0b += conj_(a[jj + 1][0]) * vb;
+          if (TWO) {
+            p1a += conj_(a[jj][1]) * va;
+            p1b += conj_(a[jj + 1][1]) * vb;
+          }
         }
       }
     }
-    P[w * 128 + l] = p0;
-    P[w * 128 + l + kWave] = p1;
+    P[w * 128 + i0] = p0a + p0b;
+    P[w * 128 + i1] = p1a + p1b;
     lds_barrier();
-    // (d) wave 0: p = tau * sum_w P, K = -tau/2 p^H v, W = p + K v
-    if (w == 0) {
-      const int c0 = l, c1 = l + kWave;
-      E q0 = P[c0] + P[128 + c0] + P[256 + c0] + P[384 + c0];
-      E q1 = P[c1] + P[128 + c1] + P[256 + c1] + P[384 + c1];
-      q0 = (c0 > k && c0 < n) ? tk * q0 : zero;
-      q1 = (c1 > k && c1 < n) ? tk * q1 : zero;
-      const E v0 = V[c0], v1 = V[c1];
-      const E pv = wave_sum(conj_(q0) * v0 + conj_(q1) * v1);
-      const E kc = (R(-0.5) * tk) * pv;
-      W[c0] = q0 + kc * v0;
-      W[c1] = q1 + kc * v1;
-    }
-    lds_barrier();
-    // (e) A -= v w^H + w v^H on my entries (zeros in V / W mask the rest)
-    const E vc0 = V[l], wc0 = W[l];
-    const E vc1 = V[l + kWave], wc1 = W[l + kWave];
+    // (d) every wave: p = tau * sum_w P, K = -tau/2 p^H v, w = p + K v
+    E q0 = P[i0] + P[128 + i0] + P[256 + i0] + P[384 + i0];
+    E q1 = P[i1] + P[128 + i1] + P[256 + i1] + P[384 + i1];
+    q0 = (i0 > k && i0 < n) ? tk * q0 : zero;
+    q1 = (i1 > k && i1 < n) ? tk * q1 : zero;
+    const E pv = dpp_sum(conj_(q0) * v0 + conj_(q1) * v1);
+    const E kc = (R(-0.5) * tk) * pv;
+    const E w0 = q0 + kc * v0, w1 = q1 + kc * v1;
+    if ((i0 & 3) == w) Wr[i0 >> 2] = w0;
+    if ((i1 & 3) == w) Wr[i1 >> 2] = w1;
+    wave_sync();
+    // (e) A -= v w^H + w v^H on my entries; publish row k+1
     const bool set0 = k < kWave - 1;  // some column l > k exists
+    const E cw0 = conj_(w0), cv0 = conj_(v0), cw1 = conj_(w1), cv1 = conj_(v1);
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) {
       if (w + 4 * (qq * QJ + QJ - 1) > k) {
 #pragma unroll
         for (int u = 0; u < QJ; ++u) {
           const int jj = qq * QJ + u;
-          const E vr = V[w + 4 * jj], wr = W[w + 4 * jj];
-          if (set0) a[jj][0] -= vr * conj_(wc0) + wr * conj_(vc0);
-          if (TWO) a[jj][1] -= vr * conj_(wc1) + wr * conj_(vc1);
+          const E vr = Vr[jj], wr = Wr[jj];
+          if (set0) a[jj][0] = rank2_update(a[jj][0], vr, cw0, wr, cv0);
+          if (TWO) a[jj][1] = rank2_update(a[jj][1], vr, cw1, wr, cv1);
+          if (w + 4 * jj == k + 1) publish(jj, k + 1);
         }
       }
-    }
-  }
-  // last diagonal entry
-  {
-    const int r = n - 1;
-    if (w == (r & 3) && l == (r & (kWave - 1))) {
-#pragma unroll
-      for (int jj = 0; jj < RJ; ++jj)
-        if (w + 4 * jj == r) dd[r] = re_((r >= kWave && TWO) ? a[jj][1] : a[jj][0]);
     }
   }
   lds_barrier();
@@ -305,7 +333,7 @@ struct GramArgs {
   int n;               // Gram size = min(M, pt)
   int scale;           // 1: divide the Gram by (M - 1) (eigen rules)
   void* H;             // np x n x n   Householder rows
-  void* DE;            // np x 2n      d | e (real type)
+  void* DE;            // np x 3n      d | e | e^2 (real type)
   void* TAU;           // np x n
 };
 
@@ -362,14 +390,14 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : NB);
   E* Xb = lds_carve<E>(cur, 128);
-  E* V = lds_carve<E>(cur, 128);
-  E* W = lds_carve<E>(cur, 128);
+  E* V = lds_carve<E>(cur, 4 * 32);
+  E* W = lds_carve<E>(cur, 4 * 32);
   E* P = lds_carve<E>(cur, 4 * 128);
   R* dd = lds_carve<R>(cur, 128);
   R* ee = lds_carve<R>(cur, 128);
   E* tt = lds_carve<E>(cur, 128);
   constexpr int RJ = NB / 4;
-  const int w = tid / kWave, l = tid % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave), l = tid % kWave;
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     int64_t sp;
@@ -524,11 +552,13 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
     E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     for (int q = tid; q < n * n; q += kNT) Hp[q] = Hl[q];
     // ---- d | e and tau
-    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 2 * n;
+    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
     E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
     for (int q = tid; q < n; q += kNT) {
+      const R ev = q < n - 1 ? ee[q] : R(0);
       DEp[q] = dd[q];
-      DEp[n + q] = q < n - 1 ? ee[q] : R(0);
+      DEp[n + q] = ev;
+      DEp[2 * n + q] = ev * ev;
       TAUp[q] = q < n - 1 ? tt[q] : E(R(0));
     }
     lds_barrier();
@@ -549,7 +579,7 @@ size_t gram_lds_bytes(int n) {
   size_t s = al(std::max(stage_b, full_b));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? SC : NB) * sizeof(int64_t));
-  s += 3 * al(128 * sizeof(E)) + al(4 * 128 * sizeof(E));
+  s += al(128 * sizeof(E)) + 2 * al(4 * 32 * sizeof(E)) + al(4 * 128 * sizeof(E));
   s += 2 * al(128 * sizeof(R)) + al(128 * sizeof(E));
   return s;
 }
@@ -573,6 +603,7 @@ struct EigArgs {
   const void* TAU;
   void* Y;           // np x n x kmax eigenvectors of the Gram (element type)
   void* Z;           // np x n x kmax tridiagonal eigenvectors (real)
+  void* SCR;         // np x 3 x n x kb inverse-iteration LU scratch (real)
   double* WC;        // np x kmax synthesis weights
   int* NCOMP;        // np
   double* THETA;     // np
@@ -595,12 +626,30 @@ __device__ __forceinline__ float fdiv_(float a, float b) { return __fdividef(a, 
 __device__ __forceinline__ double fdiv_(double a, double b) { return a / b; }
 
 // Number of eigenvalues of the tridiagonal (d, e^2) strictly below x.
+// d and e2 are wave-uniform global arrays (scalar loads); 8 entries are
+// fetched per block so the recurrence never waits on memory.
 template <class R>
-__device__ __forceinline__ int sturm(const R* d, const R* e2, int n, R x, R pivmin) {
+__device__ __forceinline__ int sturm_g(const R* __restrict__ d, const R* __restrict__ e2, int n,
+                                       R x, R pivmin) {
   R q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   int c = q < R(0);
-  for (int i = 1; i < n; ++i) {
+  int i = 1;
+  for (; i + 8 <= n; i += 8) {
+    R dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i + u];
+      ev[u] = e2[i - 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      q = (dv[u] - x) - fdiv_(ev[u], q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      c += q < R(0);
+    }
+  }
+  for (; i < n; ++i) {
     q = (d[i] - x) - fdiv_(e2[i - 1], q);
     if (fabs(q) < pivmin) q = -pivmin;
     c += q < R(0);
@@ -626,9 +675,6 @@ __device__ RT np_mean10(const RT* a, int k) {
 
 template <class R>
 struct WaveScratch {
-  R* d;
-  R* e2;
-  R* e;
   R* lo;
   R* hi;
   R* xs;
@@ -638,9 +684,10 @@ struct WaveScratch {
 };
 
 // Eigenvalues of the tridiagonal at ascending indices tgt[0..nt) by
-// multisection: the 64 lanes are split into nt groups of L = 64/nt points.
+// multisection: the 64 lanes are split into groups of L = 64/nt points.
 template <class R>
-__device__ void multisect(WaveScratch<R>& s, int n, R pivmin, R glo, R ghi, R atol, int nt) {
+__device__ void multisect(WaveScratch<R>& s, const R* __restrict__ d, const R* __restrict__ e2,
+                          int n, R pivmin, R glo, R ghi, R atol, int nt) {
   const int lane = threadIdx.x;
   for (int b0 = 0; b0 < nt; b0 += kWave) {
     const int nb = min(kWave, nt - b0);
@@ -658,9 +705,8 @@ __device__ void multisect(WaveScratch<R>& s, int n, R pivmin, R glo, R ghi, R at
       const R mid = lo + (hi - lo) * R(0.5);
       const bool done = !active || (hi - lo) <= tol || mid <= lo || mid >= hi;
       if (__all(done)) break;
-      R x = lo + (hi - lo) * (R)(sub + 1) / (R)(L + 1);
-      int c = 0;
-      if (active && !done) c = sturm(s.d, s.e2, n, x, pivmin);
+      const R x = lo + (hi - lo) * (R)(sub + 1) / (R)(L + 1);
+      const int c = sturm_g(d, e2, n, x, pivmin);
       s.xs[lane] = x;
       s.cs[lane] = c;
       wave_sync();
@@ -682,15 +728,14 @@ __device__ void multisect(WaveScratch<R>& s, int n, R pivmin, R glo, R ghi, R at
   }
 }
 
-// Solve (T - lam I) x = b for the tridiagonal (d, e) with partial pivoting;
-// b is in rhs (stride ks), the solution overwrites it.  u0/u1/u2 scratch.
+// Solve (T - lam I) x = b for the tridiagonal (d, e) with partial pivoting.
+// d / e are wave-uniform global arrays (e[n-1] = 0); b is in rhs (stride ks)
+// and the solution overwrites it; u0 / u1 / u2 (stride ks) hold U.
 template <class R>
-__device__ void tri_solve(const R* d, const R* e, int n, R lam, R* u0, R* u1, R* u2, R* rhs,
-                          int ks, R eps_piv) {
-  R c0 = d[0] - lam, c1 = n > 1 ? e[0] : R(0), c2 = R(0), r = rhs[0];
-  for (int i = 0; i < n - 1; ++i) {
-    R n0 = e[i], n1 = d[i + 1] - lam, n2 = (i + 1 < n - 1) ? e[i + 1] : R(0);
-    R nr = rhs[(i + 1) * ks];
+__device__ void tri_solve(const R* __restrict__ d, const R* __restrict__ e, int n, R lam, R* u0,
+                          R* u1, R* u2, R* rhs, int ks, R eps_piv) {
+  R c0 = d[0] - lam, c1 = e[0], c2 = R(0), r = rhs[0];
+  auto step = [&](int i, R n0, R n1, R n2, R nr) {
     if (fabs(n0) > fabs(c0)) {
       R t;
       t = c0; c0 = n0; n0 = t;
@@ -699,7 +744,7 @@ __device__ void tri_solve(const R* d, const R* e, int n, R lam, R* u0, R* u1, R*
       t = r; r = nr; nr = t;
     }
     if (c0 == R(0)) c0 = eps_piv;
-    const R f = n0 / c0;
+    const R f = fdiv_(n0, c0);
     u0[i * ks] = c0;
     u1[i * ks] = c1;
     u2[i * ks] = c2;
@@ -708,16 +753,47 @@ __device__ void tri_solve(const R* d, const R* e, int n, R lam, R* u0, R* u1, R*
     c1 = n2 - f * c2;
     c2 = R(0);
     r = nr - f * r;
+  };
+  int i = 0;
+  for (; i + 4 <= n - 1; i += 4) {
+    R dv[4], ev[4], fv[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      dv[u] = d[i + 1 + u];
+      ev[u] = e[i + u];
+      fv[u] = e[i + 1 + u];
+      bv[u] = rhs[(i + 1 + u) * ks];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(i + u, ev[u], dv[u] - lam, fv[u], bv[u]);
   }
+  for (; i < n - 1; ++i) step(i, e[i], d[i + 1] - lam, e[i + 1], rhs[(i + 1) * ks]);
   if (c0 == R(0)) c0 = eps_piv;
   u0[(n - 1) * ks] = c0;
-  rhs[(n - 1) * ks] = r;
   // back substitution
-  R z2 = R(0), z1 = r / c0;
+  R z2 = R(0), z1 = fdiv_(r, c0);
   rhs[(n - 1) * ks] = z1;
-  for (int i = n - 2; i >= 0; --i) {
-    const R z = (rhs[i * ks] - u1[i * ks] * z1 - u2[i * ks] * z2) / u0[i * ks];
-    rhs[i * ks] = z;
+  int k = n - 2;
+  for (; k - 3 >= 0; k -= 4) {
+    R a0[4], a1[4], a2[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a0[u] = u0[(k - u) * ks];
+      a1[u] = u1[(k - u) * ks];
+      a2[u] = u2[(k - u) * ks];
+      b[u] = rhs[(k - u) * ks];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const R z = fdiv_(b[u] - a1[u] * z1 - a2[u] * z2, a0[u]);
+      rhs[(k - u) * ks] = z;
+      z2 = z1;
+      z1 = z;
+    }
+  }
+  for (; k >= 0; --k) {
+    const R z = fdiv_(rhs[k * ks] - u1[k * ks] * z1 - u2[k * ks] * z2, u0[k * ks]);
+    rhs[k * ks] = z;
     z2 = z1;
     z1 = z;
   }
@@ -743,6 +819,13 @@ __device__ double shrink(int loss, double y, double beta) {
   return sqrt(fmax(t * t - 4.0 * beta, 0.0) / y);
 }
 
+constexpr int kKB = 16;  // components per eigenvector batch (registers per lane)
+
+// One wavefront per patch.  Eigenvalues by multisection on the tridiagonal,
+// the method's rule, then for the kept components: inverse iteration (one
+// lane per component, LU in scratch), Gram-Schmidt inside eigenvalue
+// clusters, and back-transformation y = H_0 ... H_{n-2} z with the vectors
+// held row-distributed in registers (lane l: rows l and l + 64).
 template <class E, class RT>
 __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   using R = typename real_of<E>::type;
@@ -751,42 +834,30 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
   WaveScratch<R> s;
-  s.d = lds_carve<R>(cur, n);
-  s.e2 = lds_carve<R>(cur, n);
-  s.e = lds_carve<R>(cur, n);
   s.lo = lds_carve<R>(cur, kWave);
   s.hi = lds_carve<R>(cur, kWave);
   s.xs = lds_carve<R>(cur, kWave);
   s.cs = lds_carve<int>(cur, kWave);
   s.tgt = lds_carve<int>(cur, n + 16);
   s.val = lds_carve<R>(cur, n + 16);
-  R* lam = lds_carve<R>(cur, n + 16);      // eigenvalue of each component
-  int* cidx = lds_carve<int>(cur, n + 16); // ascending eigen index of each component
+  R* lam = lds_carve<R>(cur, n + 16);       // eigenvalue of each component
+  int* cidx = lds_carve<int>(cur, n + 16);  // ascending eigen index of each component
   double* wc = lds_carve<double>(cur, n + 16);
-  R* u0 = lds_carve<R>(cur, (size_t)n * KB);
-  R* u1 = lds_carve<R>(cur, (size_t)n * KB);
-  R* u2 = lds_carve<R>(cur, (size_t)n * KB);
-  R* zb = lds_carve<R>(cur, (size_t)n * KB);
-  E* yb = lds_carve<E>(cur, (size_t)n * KB);
+  R* zb = lds_carve<R>(cur, (size_t)n * KB);  // z, component-major per lane: zb[i * KB + c]
   __shared__ int sh_i[4];
   __shared__ double sh_d[4];
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
-    const R* DEp = reinterpret_cast<const R*>(a.DE) + (size_t)p * 2 * n;
-    for (int i = lane; i < n; i += kWave) {
-      const R dv = DEp[i], ev = DEp[n + i];
-      s.d[i] = dv;
-      s.e[i] = ev;
-      s.e2[i] = ev * ev;
-    }
-    wave_sync();
+    const R* __restrict__ dg = reinterpret_cast<const R*>(a.DE) + (size_t)p * 3 * n;
+    const R* __restrict__ eg = dg + n;
+    const R* __restrict__ e2g = dg + 2 * n;
     // Gershgorin interval
     R glo = R(INFINITY), ghi = R(-INFINITY), emax2 = R(0);
     for (int i = lane; i < n; i += kWave) {
-      const R off = (i > 0 ? fabs(s.e[i - 1]) : R(0)) + (i < n - 1 ? fabs(s.e[i]) : R(0));
-      glo = fmin(glo, s.d[i] - off);
-      ghi = fmax(ghi, s.d[i] + off);
-      if (i < n - 1) emax2 = fmax(emax2, s.e2[i]);
+      const R off = (i > 0 ? fabs(eg[i - 1]) : R(0)) + fabs(eg[i]);
+      glo = fmin(glo, dg[i] - off);
+      ghi = fmax(ghi, dg[i] + off);
+      emax2 = fmax(emax2, e2g[i]);
     }
     for (int m = kWave / 2; m >= 1; m >>= 1) {
       glo = fmin(glo, shfl_xor_(glo, m));
@@ -810,7 +881,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         const double v = a.var_map[(sp + row_offset(a.g, r)) / a.g.T];
         acc += (a.method == PDAMD_OPTIMAL) ? sqrt(v) : v;
       }
-      prior = wave_sum(acc) / (double)a.g.M;
+      prior = dpp_sum(acc) / (double)a.g.M;
     }
 
     int ncomp = 0, rank = 0;
@@ -820,7 +891,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       const int nt = min(10, n);
       for (int j = lane; j < nt; j += kWave) s.tgt[j] = n - 1 - j;
       wave_sync();
-      multisect(s, n, pivmin, glo, ghi, atol, nt);
+      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, nt);
       if (lane == 0) {
         RT ev[10];
         for (int q = 0; q < 10; ++q) {
@@ -878,11 +949,11 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       int cand = n;
       if (tlo > 0) {
         const R x = (R)(tlo * tlo);
-        cand = n - sturm(s.d, s.e2, n, x, pivmin);
+        cand = n - sturm_g(dg, e2g, n, x, pivmin);
       }
       for (int j = lane; j < cand; j += kWave) s.tgt[j] = n - 1 - j;
       wave_sync();
-      multisect(s, n, pivmin, glo, ghi, atol, cand);
+      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, cand);
       if (lane == 0) {
         int r = 0;
         for (int j = 0; j < cand; ++j) {
@@ -916,7 +987,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           s.tgt[1] = n / 2;
         }
         wave_sync();
-        multisect(s, n, pivmin, glo, ghi, atol, nt);
+        multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, nt);
         RT med;
         if (nt == 1) {
           med = (RT)sqrt(fmax((double)s.val[0], 0.0));
@@ -933,10 +1004,10 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       int ctop = n, cbot = 0;
       {
         const double xt = cs * (1.0 + sb) * (1.0 - 1e-5);
-        if (xt > 0 && isfinite(xt)) ctop = n - sturm(s.d, s.e2, n, (R)(xt * xt), pivmin);
+        if (xt > 0 && isfinite(xt)) ctop = n - sturm_g(dg, e2g, n, (R)(xt * xt), pivmin);
         if (a.loss == PDAMD_LOSS_FRO && 1.0 - sb > 0) {
           const double xb = cs * (1.0 - sb) * (1.0 + 1e-5);
-          if (isfinite(xb)) cbot = sturm(s.d, s.e2, n, (R)(xb * xb), pivmin);
+          if (isfinite(xb)) cbot = sturm_g(dg, e2g, n, (R)(xb * xb), pivmin);
           if (!isfinite(xb)) cbot = n;
         }
         if (!isfinite(cs) || cs <= 0) {
@@ -948,7 +1019,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       for (int j = lane; j < ctop; j += kWave) s.tgt[j] = n - 1 - j;
       for (int j = lane; j < cbot; j += kWave) s.tgt[ctop + j] = j;
       wave_sync();
-      multisect(s, n, pivmin, glo, ghi, atol, ctop + cbot);
+      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, ctop + cbot);
       if (lane == 0) {
         int nc = 0, last = -1;
         for (int j = 0; j < ctop + cbot; ++j) {
@@ -978,108 +1049,144 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
 
     // ---------------- eigenvectors of the kept components
     const E* Hp = reinterpret_cast<const E*>(a.H) + (size_t)p * n * n;
-    const E* TAUp = reinterpret_cast<const E*>(a.TAU) + (size_t)p * n;
+    const E* __restrict__ TAUp = reinterpret_cast<const E*>(a.TAU) + (size_t)p * n;
     E* Yp = reinterpret_cast<E*>(a.Y) + (size_t)p * n * a.kmax;
     R* Zp = reinterpret_cast<R*>(a.Z) + (size_t)p * n * a.kmax;
+    R* scr = reinterpret_cast<R*>(a.SCR) + (size_t)p * 3 * n * KB;
     const R eps_piv = Lim<R>::ulp() * tnorm + pivmin;
     const R clus = R(1e-3) * tnorm;
+    const int i0 = lane, i1 = lane + kWave;  // rows held by this lane
     for (int c0 = 0; c0 < ncomp; c0 += KB) {
       const int kb = min(KB, ncomp - c0);
       // inverse iteration, one lane per component
+      auto solve = [&](int c) {
+        tri_solve(dg, eg, n, lam[c0 + c], scr + c, scr + n * KB + c, scr + 2 * n * KB + c,
+                  zb + c, KB, eps_piv);
+        R mx = R(0);
+        for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(zb[i * KB + c]));
+        const R inv = mx > R(0) ? R(1) / mx : R(1);
+        for (int i = 0; i < n; ++i) zb[i * KB + c] *= inv;
+      };
       if (lane < kb) {
-        const R lm = lam[c0 + lane];
         uint32_t h = 0x9E3779B9u * (uint32_t)(c0 + lane + 1);
         for (int i = 0; i < n; ++i) {
           h ^= h << 13; h ^= h >> 17; h ^= h << 5;
           zb[i * KB + lane] = R(0.5) + (R)(h & 0xFFFF) * R(1.0 / 65536.0);
         }
-        for (int it = 0; it < 3; ++it) {
-          tri_solve(s.d, s.e, n, lm, u0 + lane, u1 + lane, u2 + lane, zb + lane, KB, eps_piv);
-          R mx = R(0);
-          for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(zb[i * KB + lane]));
-          const R inv = mx > R(0) ? R(1) / mx : R(1);
-          for (int i = 0; i < n; ++i) zb[i * KB + lane] *= inv;
-        }
+        for (int it = 0; it < 3; ++it) solve(lane);
       }
       wave_sync();
-      // Gram-Schmidt within clusters (this batch and earlier batches), twice
+      // rows -> registers (lane l: rows l and l+64)
+      R z0[kKB], z1[kKB];
+#pragma unroll
+      for (int c = 0; c < kKB; ++c) {
+        z0[c] = (c < kb && i0 < n) ? zb[i0 * KB + c] : R(0);
+        z1[c] = (c < kb && i1 < n) ? zb[i1 * KB + c] : R(0);
+      }
+      // classical Gram-Schmidt, twice, inside eigenvalue clusters; one extra
+      // inverse-iteration step in between
       for (int pass = 0; pass < 2; ++pass) {
-        for (int c = 0; c < kb; ++c) {
+#pragma unroll
+        for (int c = 0; c < kKB; ++c) {
+          if (c >= kb) continue;
           const int gc = c0 + c;
-          for (int q = gc - 1; q >= 0; --q) {
+          // earlier batches (stored z in global memory)
+          for (int q = 0; q < c0; ++q) {
             if (fabs(lam[q] - lam[gc]) > clus) continue;
-            R part = R(0);
-            if (q >= c0) {
-              for (int i = lane; i < n; i += kWave) part += zb[i * KB + (q - c0)] * zb[i * KB + c];
-            } else {
-              for (int i = lane; i < n; i += kWave) part += Zp[(size_t)i * a.kmax + q] * zb[i * KB + c];
-            }
-            const R dot = wave_sum(part);
-            if (q >= c0) {
-              for (int i = lane; i < n; i += kWave) zb[i * KB + c] -= dot * zb[i * KB + (q - c0)];
-            } else {
-              for (int i = lane; i < n; i += kWave) zb[i * KB + c] -= dot * Zp[(size_t)i * a.kmax + q];
-            }
-            wave_sync();
+            const R zq0 = i0 < n ? Zp[(size_t)i0 * a.kmax + q] : R(0);
+            const R zq1 = i1 < n ? Zp[(size_t)i1 * a.kmax + q] : R(0);
+            const R dot = dpp_sum(zq0 * z0[c] + zq1 * z1[c]);
+            z0[c] -= dot * zq0;
+            z1[c] -= dot * zq1;
           }
-          R part = R(0);
-          for (int i = lane; i < n; i += kWave) part += zb[i * KB + c] * zb[i * KB + c];
-          const R nrm = sqrt(wave_sum(part));
-          const R inv = nrm > R(0) ? R(1) / nrm : R(0);
-          for (int i = lane; i < n; i += kWave) zb[i * KB + c] *= inv;
+#pragma unroll
+          for (int q = 0; q < kKB; ++q) {
+            if (q >= c) break;
+            if (fabs(lam[c0 + q] - lam[gc]) > clus) continue;
+            const R dot = dpp_sum(z0[q] * z0[c] + z1[q] * z1[c]);
+            z0[c] -= dot * z0[q];
+            z1[c] -= dot * z1[q];
+          }
+          const R inv = R(1) / sqrt(fmax(dpp_sum(z0[c] * z0[c] + z1[c] * z1[c]), Lim<R>::tiny()));
+          z0[c] *= inv;
+          z1[c] *= inv;
+        }
+        if (pass == 0) {
+#pragma unroll
+          for (int c = 0; c < kKB; ++c) {
+            if (c < kb && i0 < n) zb[i0 * KB + c] = z0[c];
+            if (c < kb && i1 < n) zb[i1 * KB + c] = z1[c];
+          }
           wave_sync();
+          if (lane < kb) solve(lane);
+          wave_sync();
+#pragma unroll
+          for (int c = 0; c < kKB; ++c) {
+            z0[c] = (c < kb && i0 < n) ? zb[i0 * KB + c] : R(0);
+            z1[c] = (c < kb && i1 < n) ? zb[i1 * KB + c] : R(0);
+          }
         }
-        if (pass == 0 && lane < kb) {  // one more inverse-iteration step
-          tri_solve(s.d, s.e, n, lam[c0 + lane], u0 + lane, u1 + lane, u2 + lane, zb + lane, KB,
-                    eps_piv);
-          R mx = R(0);
-          for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(zb[i * KB + lane]));
-          const R inv = mx > R(0) ? R(1) / mx : R(1);
-          for (int i = 0; i < n; ++i) zb[i * KB + lane] *= inv;
+      }
+      // keep z for clustering against later batches (read back by other lanes)
+      if (c0 + kb < ncomp) {
+#pragma unroll
+        for (int c = 0; c < kKB; ++c) {
+          if (c < kb && i0 < n) Zp[(size_t)i0 * a.kmax + c0 + c] = z0[c];
+          if (c < kb && i1 < n) Zp[(size_t)i1 * a.kmax + c0 + c] = z1[c];
         }
-        wave_sync();
+        __syncthreads();
       }
-      // keep z for clustering against later batches (read back through global
-      // memory by other lanes: full fence)
-      for (int q = lane; q < n * kb; q += kWave) {
-        const int i = q / kb, c = q % kb;
-        Zp[(size_t)i * a.kmax + c0 + c] = zb[i * KB + c];
+      // back-transformation y = H_0 ... H_{n-2} z, reflectors prefetched 8 ahead
+      E y0[kKB], y1[kKB];
+#pragma unroll
+      for (int c = 0; c < kKB; ++c) {
+        y0[c] = E(z0[c]);
+        y1[c] = E(z1[c]);
       }
-      if (c0 + kb < ncomp) __syncthreads();
-      // back-transformation y = H_0 ... H_{n-2} z
-      for (int q = lane; q < n * kb; q += kWave) {
-        const int i = q / kb, c = q % kb;
-        yb[i * KB + c] = E(zb[i * KB + c]);
-      }
-      wave_sync();
-      // reflectors are applied from k = n-2 down; the next one is prefetched
-      auto fetch = [&](int k, E& va, E& vb, E& tk) {
-        const int m = n - k - 1;
-        const E* vk = Hp + (size_t)k * n + k + 1;
-        tk = TAUp[k];
-        va = (lane < m) ? ((lane == 0) ? E(R(1)) : vk[lane]) : E(R(0));
-        vb = (lane + kWave < m) ? vk[lane + kWave] : E(R(0));
+      constexpr int PF = 8;
+      E va[PF], vb[PF], tv[PF];
+      auto fetch = [&](int kbase) {  // reflectors kbase, kbase-1, ..., kbase-PF+1
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int k = kbase - u;
+          if (k >= 0) {
+            const E* row = Hp + (size_t)k * n;
+            tv[u] = TAUp[k];
+            va[u] = (i0 > k + 1 && i0 < n) ? row[i0] : E(R(i0 == k + 1 ? 1 : 0));
+            vb[u] = (i1 > k + 1 && i1 < n) ? row[i1] : E(R(i1 == k + 1 ? 1 : 0));
+          } else {
+            tv[u] = E(R(0));
+            va[u] = vb[u] = E(R(0));
+          }
+        }
       };
-      E va_n, vb_n, tk_n;
-      if (n >= 2) fetch(n - 2, va_n, vb_n, tk_n);
-      for (int k = n - 2; k >= 0; --k) {
-        const E tk = tk_n, va = va_n, vb = vb_n;
-        if (k > 0) fetch(k - 1, va_n, vb_n, tk_n);
-        if (re_(tk) == R(0) && im_(tk) == R(0)) continue;
-        const int m = n - k - 1;
-        for (int c = 0; c < kb; ++c) {
-          E part = E(R(0));
-          if (lane < m) part += conj_(va) * yb[(k + 1 + lane) * KB + c];
-          if (lane + kWave < m) part += conj_(vb) * yb[(k + 1 + lane + kWave) * KB + c];
-          const E sdot = tk * wave_sum(part);
-          if (lane < m) yb[(k + 1 + lane) * KB + c] -= va * sdot;
-          if (lane + kWave < m) yb[(k + 1 + lane + kWave) * KB + c] -= vb * sdot;
+      fetch(n - 2);
+      for (int kbase = n - 2; kbase >= 0; kbase -= PF) {
+        E ca[PF], cb[PF], ct[PF];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          ca[u] = va[u];
+          cb[u] = vb[u];
+          ct[u] = tv[u];
         }
-        wave_sync();
+        if (kbase - PF >= 0) fetch(kbase - PF);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const E tk = ct[u];
+          if (re_(tk) == R(0) && im_(tk) == R(0)) continue;
+#pragma unroll
+          for (int c = 0; c < kKB; ++c) {
+            if (c >= kb) break;
+            const E sdot = tk * dpp_sum(conj_(ca[u]) * y0[c] + conj_(cb[u]) * y1[c]);
+            y0[c] -= ca[u] * sdot;
+            y1[c] -= cb[u] * sdot;
+          }
+        }
       }
-      for (int q = lane; q < n * kb; q += kWave) {
-        const int i = q / kb, c = q % kb;
-        Yp[(size_t)i * a.kmax + c0 + c] = yb[i * KB + c];
+#pragma unroll
+      for (int c = 0; c < kKB; ++c) {
+        if (c < kb && i0 < n) Yp[(size_t)i0 * a.kmax + c0 + c] = y0[c];
+        if (c < kb && i1 < n) Yp[(size_t)i1 * a.kmax + c0 + c] = y1[c];
       }
       wave_sync();
     }
@@ -1099,10 +1206,10 @@ template <class E>
 size_t trieig_lds_bytes(int n, int kb) {
   using R = typename real_of<E>::type;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  size_t s = 3 * al(n * sizeof(R)) + 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int));
+  size_t s = 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int));
   s += al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(R));
   s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(double));
-  s += 4 * al((size_t)n * kb * sizeof(R)) + al((size_t)n * kb * sizeof(E));
+  s += al((size_t)n * kb * sizeof(R));
   return s;
 }
 
@@ -1484,7 +1591,7 @@ struct Plan {
   size_t ebytes, rbytes;
   // workspace offsets
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
-      off_WC, off_NC, off_TH, off_w3, total;
+      off_WC, off_NC, off_TH, off_w3, off_SCR, total;
 };
 
 int make_plan(const pdamd_problem* P, Plan* pl) {
@@ -1543,7 +1650,7 @@ int make_plan(const pdamd_problem* P, Plan* pl) {
   pl->rbytes = e64 ? 8 : 4;
   pl->ebytes = pl->rbytes * (cplx ? 2 : 1);
   pl->kmax = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? std::min(10, n) : n;
-  pl->kb = std::min(16, pl->kmax);
+  pl->kb = std::min(kKB, pl->kmax);
   // chunk: keep the Householder block under ~1 GiB
   const size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                      3 * n * pl->ebytes + pl->kmax * 8 + 16;
@@ -1565,7 +1672,8 @@ int make_plan(const pdamd_problem* P, Plan* pl) {
   pl->off_count = take(sizeof(int64_t));
   const size_t C = (size_t)pl->chunk;
   pl->off_H = take(C * n * n * pl->ebytes);
-  pl->off_DE = take(C * 2 * n * pl->rbytes);
+  pl->off_DE = take(C * 3 * n * pl->rbytes);
+  pl->off_SCR = take(C * 3 * n * pl->kb * pl->rbytes);
   pl->off_TAU = take(C * n * pl->ebytes);
   pl->off_Y = take(C * n * pl->kmax * pl->ebytes);
   pl->off_Z = take(C * n * pl->kmax * pl->rbytes);
@@ -1705,6 +1813,7 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     ea.TAU = ga.TAU;
     ea.Y = ws + pl.off_Y;
     ea.Z = ws + pl.off_Z;
+    ea.SCR = ws + pl.off_SCR;
     ea.WC = reinterpret_cast<double*>(ws + pl.off_WC);
     ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_NC);
     ea.THETA = reinterpret_cast<double*>(ws + pl.off_TH);
