@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="auto")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
@@ -132,6 +134,14 @@ def main():
     prob.stage_ms = None
     prob.stage_launches = None
 
+    if args.calib_copy:
+        # plain copy kernel of known size: 256 MiB read + 256 MiB written
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     ms_per_step = elapsed * 1e3 / args.steps
     vf = float(np.prod(shape))
     value = vf / (ms_per_step / 1e3)
@@ -161,6 +171,16 @@ def main():
         achieved = 0.0
         roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=0.0, traffic=None, kernel=dom)
+
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by a copy of
+    # known size); null when no summary exists for this kernel.
+    tpath = os.path.join(REPO, "profiles", f"hbm_traffic_{WORKLOAD.split('_p5')[0]}.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("kernel") == roof.get("kernel"):
+            roof["traffic"] = tr["bytes_per_dispatch"]
+            roof["traffic_source"] = tr["source"]
 
     cpu = None
     if not args.no_cpu:
