@@ -154,6 +154,10 @@ __global__ __launch_bounds__(kNT) void compact_kernel(const uint8_t* __restrict_
 // 2. Gram + Householder tridiagonalisation (utils.py:102-137 / :58-78)
 // ===========================================================================
 
+// Offset of packed Householder row k (columns k+1 .. n-1) in an n x n reduction.
+__host__ __device__ __forceinline__ int hh_row(int k, int n) { return k * (n - 1) - (k * (k - 1)) / 2; }
+__host__ __device__ __forceinline__ size_t hh_size(int n) { return (size_t)n * (n - 1) / 2; }
+
 // a - x*y - z*t as explicit fused multiply-adds.
 __device__ __forceinline__ float rank2_update(float a, float x, float y, float z, float t) {
   return fmaf(-z, t, fmaf(-x, y, a));
@@ -193,8 +197,9 @@ __device__ __forceinline__ cpx<R> rank2_update(cpx<R> a, cpx<R> x, cpx<R> y, cpx
 //            it for the next step.
 // v and w are zero at indices <= k and >= n, so no entry needs masking; whole
 // quarters of inactive rows are skipped with wave-uniform branches.
-// Outputs: dd / ee (real tridiagonal), tt (tau_k), and v_k in row k of Hrow
-// (LDS, row stride n, columns k+1 .. n-1, v_k[0] = 1).
+// Outputs: dd / ee (real tridiagonal), tt (tau_k), and v_k in packed row k of
+// Hrow (LDS; row k starts at hh_row(k, n) and holds columns k+1 .. n-1,
+// v_k[0] = 1).
 template <class E, int RJ>
 __device__ void tridiag_reg(E (&a)[RJ][2], int n, typename real_of<E>::type* dd,
                             typename real_of<E>::type* ee, E* tt, E* X, E* Vall, E* Wall, E* P,
@@ -239,8 +244,9 @@ __device__ void tridiag_reg(E (&a)[RJ][2], int n, typename real_of<E>::type* dd,
     if ((i0 & 3) == w) Vr[i0 >> 2] = v0;
     if ((i1 & 3) == w) Vr[i1 >> 2] = v1;
     if (w == 0) {
-      if (i0 > k && i0 < n) Hrow[(size_t)k * n + i0] = v0;
-      if (i1 > k && i1 < n) Hrow[(size_t)k * n + i1] = v1;
+      const int pk = hh_row(k, n) - (k + 1);  // packed row k, absolute column index
+      if (i0 > k && i0 < n) Hrow[pk + i0] = v0;
+      if (i1 > k && i1 < n) Hrow[pk + i1] = v1;
       if (l == 0) {
         tt[k] = tk;
         ee[k] = beta;
@@ -358,7 +364,7 @@ template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
   return {a.x / (R)n, a.y / (R)n};
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
+template <class IN, class E, bool TSIDE, int TILE, bool XG>
 __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
   using R = typename real_of<E>::type;
@@ -379,14 +385,14 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
   // row/column ownership, and the Householder vectors (row k, columns > k).
   const size_t stage_b = ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16 +
                          (SAME ? 0 : ((size_t)SC * LDS_W * sizeof(E) + 15) / 16 * 16);
-  const size_t full_b = (size_t)n * lda * sizeof(E);
+  const size_t full_b = XG ? 0 : (size_t)n * lda * sizeof(E);
+  const size_t hh_b = hh_size(n) * sizeof(E);
   char* U = cur;
-  cur += (std::max(stage_b, full_b) + 15) / 16 * 16;
+  cur += (std::max(std::max(stage_b, full_b), hh_b) + 15) / 16 * 16;
   IN* Sraw = reinterpret_cast<IN*>(U);
   E* Sc = SAME ? reinterpret_cast<E*>(U)
                : reinterpret_cast<E*>(U + ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16);
-  E* A = reinterpret_cast<E*>(U);   // exchange (row stride lda)
-  E* Hl = reinterpret_cast<E*>(U);  // Householder rows (row stride n)
+  E* Hl = reinterpret_cast<E*>(U);  // packed Householder rows
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : NB);
   E* Xb = lds_carve<E>(cur, 128);
@@ -520,8 +526,12 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
       }
     }
 
-    // ---- Gram -> Householder ownership through LDS
+    // ---- Gram -> Householder ownership, through LDS (or through this patch's
+    //      global H block when the LDS copy does not fit: XG)
     //      (optionally / (M-1), as the reference's eigen path)
+    E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+    E* A = XG ? Hp : reinterpret_cast<E*>(U);
+    const int ldx = XG ? n : lda;
 #pragma unroll
     for (int i = 0; i < TILE; ++i) {
       const int r = ty + 16 * i;
@@ -531,26 +541,27 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
         if (r < n && c < n) {
           E v = acc[i][j];
           if (a.scale) v = div_n(v, M - 1);
-          A[(size_t)r * lda + c] = v;
+          A[(size_t)r * ldx + c] = v;
         }
       }
     }
-    lds_barrier();
+    if (XG) __syncthreads();  // global exchange: full fence
+    else lds_barrier();
     E areg[RJ][2];
 #pragma unroll
     for (int jj = 0; jj < RJ; ++jj) {
       const int r = w + 4 * jj;
-      areg[jj][0] = (r < n && l < n) ? A[(size_t)r * lda + l] : E(R(0));
-      areg[jj][1] = (RJ * 4 > 64 && r < n && l + kWave < n) ? A[(size_t)r * lda + l + kWave]
+      areg[jj][0] = (r < n && l < n) ? A[(size_t)r * ldx + l] : E(R(0));
+      areg[jj][1] = (RJ * 4 > 64 && r < n && l + kWave < n) ? A[(size_t)r * ldx + l + kWave]
                                                              : E(R(0));
     }
-    lds_barrier();
+    if (XG) __syncthreads();
+    else lds_barrier();
 
     tridiag_reg<E, RJ>(areg, n, dd, ee, tt, Xb, V, W, P, Hl);
 
-    // ---- Householder rows -> global
-    E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
-    for (int q = tid; q < n * n; q += kNT) Hp[q] = Hl[q];
+    // ---- packed Householder rows -> global
+    for (int q = tid; q < (int)hh_size(n); q += kNT) Hp[q] = Hl[q];
     // ---- d | e and tau
     R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
     E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
@@ -565,7 +576,7 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
   }
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
+template <class IN, class E, bool TSIDE, int TILE, bool XG>
 size_t gram_lds_bytes(int n) {
   constexpr int NB = 16 * TILE;
   constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
@@ -575,8 +586,9 @@ size_t gram_lds_bytes(int n) {
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
   const int lda = n | 1;
   const size_t stage_b = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
-  const size_t full_b = (size_t)n * lda * sizeof(E);
-  size_t s = al(std::max(stage_b, full_b));
+  const size_t full_b = XG ? 0 : (size_t)n * lda * sizeof(E);
+  const size_t hh_b = hh_size(n) * sizeof(E);
+  size_t s = al(std::max(std::max(stage_b, full_b), hh_b));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? SC : NB) * sizeof(int64_t));
   s += al(128 * sizeof(E)) + 2 * al(4 * 32 * sizeof(E)) + al(4 * 128 * sizeof(E));
@@ -1150,7 +1162,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         for (int u = 0; u < PF; ++u) {
           const int k = kbase - u;
           if (k >= 0) {
-            const E* row = Hp + (size_t)k * n;
+            const E* row = Hp + hh_row(k, n) - (k + 1);  // indexed by absolute column
             tv[u] = TAUp[k];
             va[u] = (i0 > k + 1 && i0 < n) ? row[i0] : E(R(i0 == k + 1 ? 1 : 0));
             vb[u] = (i1 > k + 1 && i1 < n) ? row[i1] : E(R(i1 == k + 1 ? 1 : 0));
@@ -1724,16 +1736,25 @@ int grid_for(int64_t np, int per_cu) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(np, (int64_t)n_cu * per_cu));
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
-int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n);
+template <class IN, class E, bool TSIDE, int TILE, bool XG>
+int launch_gram_x(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE, XG>(pl.n);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = gram_tridiag_kernel<IN, E, TSIDE, TILE>;
+  auto kern = gram_tridiag_kernel<IN, E, TSIDE, TILE, XG>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, (int)((160 * 1024) / lds));
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
+}
+
+// The Gram moves to the Householder layout through LDS when it fits there,
+// otherwise through the patch's global H block.
+template <class IN, class E, bool TSIDE, int TILE>
+int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  if (gram_lds_bytes<IN, E, TSIDE, TILE, false>(pl.n) <= 160 * 1024)
+    return launch_gram_x<IN, E, TSIDE, TILE, false>(pl, vol, ga, st);
+  return launch_gram_x<IN, E, TSIDE, TILE, true>(pl, vol, ga, st);
 }
 
 template <class IN, class E, bool TSIDE>
