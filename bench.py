@@ -36,17 +36,15 @@ WORKLOAD = "mppca_80x80x48x300_p5_o4_ellipsoid_weighted"
 
 
 def algorithmic_work(n, N, M, T, ranks):
-    """Per-launch algorithmic work of each stage for the selected patches.
+    """Algorithmic flops of each per-patch kernel summed over the selected patches.
 
-    gram_tridiag: Gram n(n+1)N (SYRK count, min side) + Householder
-                  tridiagonalisation 4/3 n^3, real flops per patch.
-    synth:        4 M T r flops per patch (two rank-r products).
-    Bytes (HBM, ideal): input read once + output and weights written once.
+    gram:    n(n+1)N real flops per patch (SYRK count of the min-side Gram).
+    tridiag: 4/3 n^3 per patch (Householder tridiagonalisation).
+    synth:   4 M T r per patch (two rank-r products).
     """
     P = len(ranks)
-    f_gram = P * (n * (n + 1) * N + 4.0 / 3.0 * n**3)
-    f_syn = float(4.0 * M * T * np.sum(ranks))
-    return f_gram, f_syn
+    return dict(gram=P * n * (n + 1) * N, tridiag=P * 4.0 / 3.0 * n**3,
+                synth=float(4.0 * M * T * np.sum(ranks)))
 
 
 def cpu_baseline(data, mask, ps, po, sel, stride):
@@ -121,8 +119,8 @@ def main():
         run.run()
     torch.cuda.synchronize()
 
-    stage_ms = (ctypes.c_float * 5)()
-    stage_n = (ctypes.c_int64 * 5)()
+    stage_ms = (ctypes.c_float * len(N.STAGES))()
+    stage_n = (ctypes.c_int64 * len(N.STAGES))()
     prob.stage_ms = ctypes.cast(stage_ms, ctypes.c_void_p).value
     prob.stage_launches = ctypes.cast(stage_n, ctypes.c_void_p).value
     torch.cuda.synchronize()
@@ -150,27 +148,22 @@ def main():
     sel = info["index"]
     M, T = 125, shape[-1]
     n, Nn = min(M, T), max(M, T)
-    f_gram, f_syn = algorithmic_work(n, Nn, M, T, ranks)
+    work = algorithmic_work(n, Nn, M, T, ranks)
+    NST = len(N.STAGES)
     stages = {name: dict(ms_per_step=stage_ms[i] / args.steps,
                          launches_per_step=stage_n[i] / args.steps)
               for i, name in enumerate(N.STAGES)}
     dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
-    # per-launch average of the dominant kernel
-    dom_ms = stage_ms[N.STAGES.index(dom)] / max(stage_n[N.STAGES.index(dom)], 1)
-    launches = max(stage_n[N.STAGES.index(dom)] / args.steps, 1)
-    if dom == "gram_tridiag":
-        achieved = f_gram / launches / (dom_ms / 1e3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="gram_tridiag_kernel")
-    elif dom == "synth":
-        achieved = f_syn / launches / (dom_ms / 1e3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="synth_kernel")
-    else:
-        # trieig: O(n) per Sturm count; report against FP32 peak on its algorithmic flops
-        achieved = 0.0
-        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=0.0, traffic=None, kernel=dom)
+    di = N.STAGES.index(dom)
+    dom_ms = stage_ms[di] / max(stage_n[di], 1)          # average launch duration
+    launches = max(stage_n[di] / args.steps, 1)          # launches per step
+    kname = {"gram": "gram_kernel", "tridiag": "tridiag_kernel", "synth": "synth_kernel",
+             "trieig": "trieig_kernel"}.get(dom, dom)
+    flops = work.get(dom)
+    achieved = (flops / launches / (dom_ms / 1e3) / 1e12) if flops else 0.0
+    roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel=kname,
+                avg_launch_ms=dom_ms)
 
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by a copy of

@@ -122,11 +122,12 @@ typedef struct pdamd_problem {
 /* stages for pdamd_problem.stage_ms */
 enum {
   PDAMD_STAGE_SELECT = 0,   /* mask window + select + scan + compact          */
-  PDAMD_STAGE_GRAM = 1,     /* gram_tridiag_kernel: gather, centre, Gram, Householder */
+  PDAMD_STAGE_GRAM = 1,     /* gram_kernel: gather, centre, Gram               */
   PDAMD_STAGE_EIG = 2,      /* trieig_kernel: eigenvalues, rule, eigenvectors  */
   PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate   */
   PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask         */
-  PDAMD_NSTAGES = 5
+  PDAMD_STAGE_TRIDIAG = 5,  /* tridiag_kernel: Householder tridiagonalisation  */
+  PDAMD_NSTAGES = 6
 };
 
 /* Bytes of device workspace pdamd_denoise needs for this problem. */

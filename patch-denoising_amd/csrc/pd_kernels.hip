@@ -364,8 +364,8 @@ template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
   return {a.x / (R)n, a.y / (R)n};
 }
 
-template <class IN, class E, bool TSIDE, int TILE, bool XG>
-__global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict__ vol,
+template <class IN, class E, bool TSIDE, int TILE>
+__global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
   using R = typename real_of<E>::type;
   constexpr int NB = 16 * TILE;  // padded Gram size
@@ -374,36 +374,15 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
   constexpr int LDS_W = NB + 1;  // staging row length (padded)
   const Geom& g = a.g;
   const int n = a.n;
-  const int lda = n | 1;
   const int tid = threadIdx.x;
   const int ty = tid / 16, tx = tid % 16;
 
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  // Region U is used in turn by the staging buffers (Gram accumulation), the
-  // full-matrix exchange from the 16x16 accumulation tiling to the Householder
-  // row/column ownership, and the Householder vectors (row k, columns > k).
-  const size_t stage_b = ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16 +
-                         (SAME ? 0 : ((size_t)SC * LDS_W * sizeof(E) + 15) / 16 * 16);
-  const size_t full_b = XG ? 0 : (size_t)n * lda * sizeof(E);
-  const size_t hh_b = hh_size(n) * sizeof(E);
-  char* U = cur;
-  cur += (std::max(std::max(stage_b, full_b), hh_b) + 15) / 16 * 16;
-  IN* Sraw = reinterpret_cast<IN*>(U);
-  E* Sc = SAME ? reinterpret_cast<E*>(U)
-               : reinterpret_cast<E*>(U + ((size_t)SC * LDS_W * sizeof(IN) + 15) / 16 * 16);
-  E* Hl = reinterpret_cast<E*>(U);  // packed Householder rows
+  IN* Sraw = lds_carve<IN>(cur, (size_t)SC * LDS_W);
+  E* Sc = SAME ? reinterpret_cast<E*>(Sraw) : lds_carve<E>(cur, (size_t)SC * LDS_W);
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : NB);
-  E* Xb = lds_carve<E>(cur, 128);
-  E* V = lds_carve<E>(cur, 4 * 32);
-  E* W = lds_carve<E>(cur, 4 * 32);
-  E* P = lds_carve<E>(cur, 4 * 128);
-  R* dd = lds_carve<R>(cur, 128);
-  R* ee = lds_carve<R>(cur, 128);
-  E* tt = lds_carve<E>(cur, 128);
-  constexpr int RJ = NB / 4;
-  const int w = __builtin_amdgcn_readfirstlane(tid / kWave), l = tid % kWave;
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     int64_t sp;
@@ -526,12 +505,9 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
       }
     }
 
-    // ---- Gram -> Householder ownership, through LDS (or through this patch's
-    //      global H block when the LDS copy does not fit: XG)
-    //      (optionally / (M-1), as the reference's eigen path)
-    E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
-    E* A = XG ? Hp : reinterpret_cast<E*>(U);
-    const int ldx = XG ? n : lda;
+    // ---- Gram -> this patch's H block (row-major, optionally / (M-1) as the
+    //      reference's eigen path)
+    E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
 #pragma unroll
     for (int i = 0; i < TILE; ++i) {
       const int r = ty + 16 * i;
@@ -541,28 +517,57 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
         if (r < n && c < n) {
           E v = acc[i][j];
           if (a.scale) v = div_n(v, M - 1);
-          A[(size_t)r * ldx + c] = v;
+          G[(size_t)r * n + c] = v;
         }
       }
     }
-    if (XG) __syncthreads();  // global exchange: full fence
-    else lds_barrier();
+  }
+}
+
+template <class IN, class E, bool TSIDE, int TILE>
+size_t gram_lds_bytes(int n) {
+  constexpr int NB = 16 * TILE;
+  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
+  constexpr int LDS_W = NB + 1;
+  constexpr bool SAME = std::is_same<IN, E>::value;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  size_t s = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
+  s += al(NB * sizeof(IN));
+  s += al((TSIDE ? SC : NB) * sizeof(int64_t));
+  return s;
+}
+
+// Householder tridiagonalisation of each patch's Gram (read from its H block),
+// leaving d | e | e^2, tau and the packed reflectors in that block.
+template <class E, int RJ>
+__global__ __launch_bounds__(kNT) void tridiag_kernel(GramArgs a) {
+  using R = typename real_of<E>::type;
+  const int n = a.n;
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave), l = tid % kWave;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  E* Hl = lds_carve<E>(cur, hh_size(n));
+  E* Xb = lds_carve<E>(cur, 128);
+  E* V = lds_carve<E>(cur, 4 * 32);
+  E* W = lds_carve<E>(cur, 4 * 32);
+  E* P = lds_carve<E>(cur, 4 * 128);
+  R* dd = lds_carve<R>(cur, 128);
+  R* ee = lds_carve<R>(cur, 128);
+  E* tt = lds_carve<E>(cur, 128);
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     E areg[RJ][2];
 #pragma unroll
     for (int jj = 0; jj < RJ; ++jj) {
       const int r = w + 4 * jj;
-      areg[jj][0] = (r < n && l < n) ? A[(size_t)r * ldx + l] : E(R(0));
-      areg[jj][1] = (RJ * 4 > 64 && r < n && l + kWave < n) ? A[(size_t)r * ldx + l + kWave]
+      areg[jj][0] = (r < n && l < n) ? Hp[(size_t)r * n + l] : E(R(0));
+      areg[jj][1] = (RJ * 4 > 64 && r < n && l + kWave < n) ? Hp[(size_t)r * n + l + kWave]
                                                              : E(R(0));
     }
-    if (XG) __syncthreads();
-    else lds_barrier();
-
     tridiag_reg<E, RJ>(areg, n, dd, ee, tt, Xb, V, W, P, Hl);
-
-    // ---- packed Householder rows -> global
+    __syncthreads();  // every read of the Gram precedes the overwrite below
     for (int q = tid; q < (int)hh_size(n); q += kNT) Hp[q] = Hl[q];
-    // ---- d | e and tau
     R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
     E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
     for (int q = tid; q < n; q += kNT) {
@@ -576,24 +581,12 @@ __global__ __launch_bounds__(kNT) void gram_tridiag_kernel(const IN* __restrict_
   }
 }
 
-template <class IN, class E, bool TSIDE, int TILE, bool XG>
-size_t gram_lds_bytes(int n) {
-  constexpr int NB = 16 * TILE;
-  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
-  constexpr int LDS_W = NB + 1;
-  constexpr bool SAME = std::is_same<IN, E>::value;
+template <class E>
+size_t tridiag_lds_bytes(int n) {
   using R = typename real_of<E>::type;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  const int lda = n | 1;
-  const size_t stage_b = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
-  const size_t full_b = XG ? 0 : (size_t)n * lda * sizeof(E);
-  const size_t hh_b = hh_size(n) * sizeof(E);
-  size_t s = al(std::max(std::max(stage_b, full_b), hh_b));
-  s += al(NB * sizeof(IN));
-  s += al((TSIDE ? SC : NB) * sizeof(int64_t));
-  s += al(128 * sizeof(E)) + 2 * al(4 * 32 * sizeof(E)) + al(4 * 128 * sizeof(E));
-  s += 2 * al(128 * sizeof(R)) + al(128 * sizeof(E));
-  return s;
+  return al(hh_size(n) * sizeof(E)) + al(128 * sizeof(E)) + 2 * al(4 * 32 * sizeof(E)) +
+         al(4 * 128 * sizeof(E)) + 2 * al(128 * sizeof(R)) + al(128 * sizeof(E));
 }
 
 // ===========================================================================
@@ -1736,25 +1729,29 @@ int grid_for(int64_t np, int per_cu) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(np, (int64_t)n_cu * per_cu));
 }
 
-template <class IN, class E, bool TSIDE, int TILE, bool XG>
-int launch_gram_x(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE, XG>(pl.n);
+template <class IN, class E, bool TSIDE, int TILE>
+int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = gram_tridiag_kernel<IN, E, TSIDE, TILE, XG>;
+  auto kern = gram_kernel<IN, E, TSIDE, TILE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, (int)((160 * 1024) / lds));
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }
 
-// The Gram moves to the Householder layout through LDS when it fits there,
-// otherwise through the patch's global H block.
-template <class IN, class E, bool TSIDE, int TILE>
-int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  if (gram_lds_bytes<IN, E, TSIDE, TILE, false>(pl.n) <= 160 * 1024)
-    return launch_gram_x<IN, E, TSIDE, TILE, false>(pl, vol, ga, st);
-  return launch_gram_x<IN, E, TSIDE, TILE, true>(pl, vol, ga, st);
+template <class E, int RJ>
+int launch_tridiag(const Plan& pl, GramArgs ga, hipStream_t st) {
+  const size_t lds = tridiag_lds_bytes<E>(pl.n);
+  if (lds > 160 * 1024)
+    return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes exceeds 160 KiB", lds);
+  auto kern = tridiag_kernel<E, RJ>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+  kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(ga);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
 }
 
 template <class IN, class E, bool TSIDE>
@@ -1763,6 +1760,15 @@ int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t s
     case 2: return launch_gram<IN, E, TSIDE, 2>(pl, vol, ga, st);
     case 4: return launch_gram<IN, E, TSIDE, 4>(pl, vol, ga, st);
     default: return launch_gram<IN, E, TSIDE, 8>(pl, vol, ga, st);
+  }
+}
+
+template <class E>
+int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
+  switch (pl.tile) {
+    case 2: return launch_tridiag<E, 8>(pl, ga, st);
+    case 4: return launch_tridiag<E, 16>(pl, ga, st);
+    default: return launch_tridiag<E, 32>(pl, ga, st);
   }
 }
 
@@ -1810,6 +1816,10 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     tm.begin(PDAMD_STAGE_GRAM);
     int rc = pl.tside ? launch_gram_tile<IN, E, true>(pl, P->input_dev, ga, st)
                       : launch_gram_tile<IN, E, false>(pl, P->input_dev, ga, st);
+    tm.end();
+    if (rc) return rc;
+    tm.begin(PDAMD_STAGE_TRIDIAG);
+    rc = launch_tridiag_tile<E>(pl, ga, st);
     tm.end();
     if (rc) return rc;
     EigArgs ea;

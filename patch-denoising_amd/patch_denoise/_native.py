@@ -64,7 +64,7 @@ class Problem(ctypes.Structure):
     ]
 
 
-STAGES = ("select", "gram_tridiag", "trieig", "synth", "finalize")
+STAGES = ("select", "gram", "trieig", "synth", "finalize", "tridiag")
 
 
 EXPORTS = (
