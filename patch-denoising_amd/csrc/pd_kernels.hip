@@ -332,6 +332,8 @@ __device__ __forceinline__ T seq_add(T s, const T* a, int count, int stride) {
   return s;
 }
 
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
 struct GramArgs {
   Geom g;
   const int32_t* idx;  // selected patch indices (this chunk)
@@ -365,7 +367,7 @@ template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
 }
 
 template <class IN, class E, bool TSIDE, int TILE>
-__global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
+__global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
   using R = typename real_of<E>::type;
   constexpr int NB = 16 * TILE;  // padded Gram size
@@ -376,13 +378,18 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
   const int n = a.n;
   const int tid = threadIdx.x;
   const int ty = tid / 16, tx = tid % 16;
+  // FP32 real Grams of size 65..128 use MFMA 32x32x2: wave w owns the 64x64
+  // block (w >> 1, w & 1), i.e. 2x2 tiles of 32x32 (16 accumulators per lane each).
+  constexpr bool MF = std::is_same<E, float>::value && TILE == 8;
+  const int wv = tid / kWave, lane = tid % kWave;
+  const int bi = wv >> 1, bj = wv & 1, lhi = lane >> 5, llo = lane & 31;
 
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
   IN* Sraw = lds_carve<IN>(cur, (size_t)SC * LDS_W);
   E* Sc = SAME ? reinterpret_cast<E*>(Sraw) : lds_carve<E>(cur, (size_t)SC * LDS_W);
   IN* mean = lds_carve<IN>(cur, NB);
-  int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : NB);
+  int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? g.M : NB);  // row offsets of the patch
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     int64_t sp;
@@ -391,11 +398,35 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
     const IN* X = vol + sp + t0;
     const int M = g.M, T = g.pt;
 
-    E acc[TILE][TILE];
+    E acc[MF ? 1 : TILE][MF ? 1 : TILE];
+    floatx16 mc[2][2];
+    if constexpr (MF) {
 #pragma unroll
-    for (int i = 0; i < TILE; ++i)
+      for (int a2 = 0; a2 < 2; ++a2)
 #pragma unroll
-      for (int j = 0; j < TILE; ++j) acc[i][j] = E(R(0));
+        for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+          for (int q = 0; q < 16; ++q) mc[a2][b2][q] = 0.0f;
+    } else {
+#pragma unroll
+      for (int i = 0; i < TILE; ++i)
+#pragma unroll
+        for (int j = 0; j < TILE; ++j) acc[i][j] = E(R(0));
+    }
+    // one reduction index k of the staged S[k][i] (i = Gram row/column index)
+    auto accum = [&](const E* Sk, int kcount) {
+      if constexpr (MF) {
+        for (int k = 0; k < kcount; k += 2) {
+          const float* row = reinterpret_cast<const float*>(Sk) + (k + lhi) * LDS_W;
+          const float a0 = row[64 * bi + llo], a1 = row[64 * bi + 32 + llo];
+          const float b0 = row[64 * bj + llo], b1 = row[64 * bj + 32 + llo];
+          mc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, mc[0][0], 0, 0, 0);
+          mc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, mc[0][1], 0, 0, 0);
+          mc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, mc[1][0], 0, 0, 0);
+          mc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, mc[1][1], 0, 0, 0);
+        }
+      }
+    };
 
     constexpr int NL = (SC * NB + kNT - 1) / kNT;  // staged elements per thread
     IN buf[NL];
@@ -432,28 +463,34 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
           Sc[t * LDS_W + r] = (t < tc && r < M) ? cvt<E>(v - mean[t]) : E(R(0));
         }
         lds_barrier();
-        for (int t = 0; t < tc; ++t) {
-          E xa[TILE], xb[TILE];
+        if constexpr (MF) {
+          accum(Sc, tc);  // rows t >= tc of the staging are zero
+        } else {
+          for (int t = 0; t < tc; ++t) {
+            E xa[TILE], xb[TILE];
 #pragma unroll
-          for (int i = 0; i < TILE; ++i) xa[i] = Sc[t * LDS_W + ty + 16 * i];
+            for (int i = 0; i < TILE; ++i) xa[i] = Sc[t * LDS_W + ty + 16 * i];
 #pragma unroll
-          for (int j = 0; j < TILE; ++j) xb[j] = conj_(Sc[t * LDS_W + tx + 16 * j]);
+            for (int j = 0; j < TILE; ++j) xb[j] = conj_(Sc[t * LDS_W + tx + 16 * j]);
 #pragma unroll
-          for (int i = 0; i < TILE; ++i)
+            for (int i = 0; i < TILE; ++i)
 #pragma unroll
-            for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+              for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+          }
         }
         lds_barrier();
       }
     } else {
       // ---- T-side: G = Xc^H Xc (T x T); pass 1 = per-frame mean over all rows
+      for (int r = tid; r < M; r += kNT) roff[r] = row_offset(g, r);
+      lds_barrier();
       auto load = [&](int r0) {
         const int rc = min(SC, M - r0);
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
           const int q = tid + u * kNT;
           const int r = q / NB, t = q % NB;
-          buf[u] = (q < SC * NB && r < rc && t < T) ? X[row_offset(g, r0 + r) + t] : IN(0);
+          buf[u] = (q < SC * NB && r < rc && t < T) ? X[roff[r0 + r] + t] : IN(0);
         }
       };
       auto store = [&]() {
@@ -490,16 +527,20 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
           Sc[r * LDS_W + t] = c;
         }
         lds_barrier();
-        for (int r = 0; r < rc; ++r) {
-          E xa[TILE], xb[TILE];
+        if constexpr (MF) {
+          accum(Sc, rc);  // rows r >= rc of the staging are zero
+        } else {
+          for (int r = 0; r < rc; ++r) {
+            E xa[TILE], xb[TILE];
 #pragma unroll
-          for (int i = 0; i < TILE; ++i) xa[i] = conj_(Sc[r * LDS_W + ty + 16 * i]);
+            for (int i = 0; i < TILE; ++i) xa[i] = conj_(Sc[r * LDS_W + ty + 16 * i]);
 #pragma unroll
-          for (int j = 0; j < TILE; ++j) xb[j] = Sc[r * LDS_W + tx + 16 * j];
+            for (int j = 0; j < TILE; ++j) xb[j] = Sc[r * LDS_W + tx + 16 * j];
 #pragma unroll
-          for (int i = 0; i < TILE; ++i)
+            for (int i = 0; i < TILE; ++i)
 #pragma unroll
-            for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+              for (int j = 0; j < TILE; ++j) acc[i][j] += xa[i] * xb[j];
+          }
         }
         lds_barrier();
       }
@@ -508,6 +549,23 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
     // ---- Gram -> this patch's H block (row-major, optionally / (M-1) as the
     //      reference's eigen path)
     E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+    if constexpr (MF) {
+      // 32x32 MFMA output: accumulator q of lane l is C[8(q/4) + 4(l/32) + q%4][l%32]
+#pragma unroll
+      for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int r = 64 * bi + 32 * a2 + 8 * (q >> 2) + 4 * lhi + (q & 3);
+            const int c = 64 * bj + 32 * b2 + llo;
+            if (r < n && c < n) {
+              float v = mc[a2][b2][q];
+              if (a.scale) v = div_n(v, M - 1);
+              reinterpret_cast<float*>(G)[(size_t)r * n + c] = v;
+            }
+          }
+    } else {
 #pragma unroll
     for (int i = 0; i < TILE; ++i) {
       const int r = ty + 16 * i;
@@ -521,11 +579,12 @@ __global__ __launch_bounds__(kNT) void gram_kernel(const IN* __restrict__ vol,
         }
       }
     }
+    }
   }
 }
 
 template <class IN, class E, bool TSIDE, int TILE>
-size_t gram_lds_bytes(int n) {
+size_t gram_lds_bytes(int n, int M) {
   constexpr int NB = 16 * TILE;
   constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
   constexpr int LDS_W = NB + 1;
@@ -533,7 +592,7 @@ size_t gram_lds_bytes(int n) {
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
   size_t s = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
   s += al(NB * sizeof(IN));
-  s += al((TSIDE ? SC : NB) * sizeof(int64_t));
+  s += al((TSIDE ? (size_t)M : (size_t)NB) * sizeof(int64_t));
   return s;
 }
 
@@ -1731,7 +1790,7 @@ int grid_for(int64_t np, int per_cu) {
 
 template <class IN, class E, bool TSIDE, int TILE>
 int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n);
+  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n, pl.g.M);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = gram_kernel<IN, E, TSIDE, TILE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
