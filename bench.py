@@ -171,8 +171,9 @@ def main():
     tpath = os.path.join(REPO, "profiles", f"hbm_traffic_{WORKLOAD.split('_p5')[0]}.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("kernel") == roof.get("kernel"):
-            roof["traffic"] = tr["bytes_per_dispatch"]
+        k = tr.get("kernels", {}).get(roof["kernel"])
+        if k is not None:
+            roof["traffic"] = k["bytes_per_dispatch"]
             roof["traffic_source"] = tr["source"]
 
     cpu = None
