@@ -77,6 +77,62 @@ def cpu_baseline(data, mask, ps, po, sel, stride):
                 full_estimate_s=full, blas=blas)
 
 
+def bench_differentiable(steps):
+    """Forward and forward+backward of the differentiable optimal-shrinkage
+    path on the rank-6 64x64x32x120 workload (patch 5, overlap 4, Frobenius
+    loss, per-patch lambda = 1, loss = MSE to the clean signal)."""
+    import torch
+
+    from patch_denoise import _native as N
+    from patch_denoise import workloads
+    from patch_denoise.differentiable import _Spec, optimal_shrinkage
+
+    noisy, clean = workloads.config5()
+    x = torch.from_numpy(noisy).cuda().requires_grad_(True)
+    tgt = torch.from_numpy(clean).cuda()
+    spec = _Spec(noisy.shape, np.float32, 5, 4)
+    lam = torch.ones(spec.grid4[:3], dtype=torch.float32, device="cuda", requires_grad=True)
+
+    def fwd():
+        return optimal_shrinkage(x, lam, 5, 4, spec=spec)
+
+    def step():
+        x.grad = None
+        lam.grad = None
+        loss = ((fwd() - tgt) ** 2).mean()
+        loss.backward()
+
+    with torch.no_grad():
+        fwd()
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        for _ in range(steps):
+            fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / steps
+    ms = (ctypes.c_float * len(N.STAGES))()
+    cnt = (ctypes.c_int64 * len(N.STAGES))()
+    spec.timers = (ms, cnt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / steps
+    spec.timers = None
+    ok = bool(torch.isfinite(x.grad).all() and torch.isfinite(lam.grad).all())
+    vf = float(np.prod(noisy.shape))
+    return dict(workload="optimal_fro_64x64x32x120_p5_o4_lambda_mse", dtype_in="f32",
+                eigen_dtype="f64", patches=spec.selected, steps=steps,
+                fwd_ms=t_f * 1e3, fwd_bwd_ms=t_fb * 1e3, ratio=t_fb / t_f,
+                fwd_voxel_frames_s=vf / t_f, fwd_bwd_voxel_frames_s=vf / t_fb,
+                grads_finite=ok,
+                stage_ms_per_step={name: ms[i] / steps for i, name in enumerate(N.STAGES)
+                                   if cnt[i]})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +141,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="auto")
+    ap.add_argument("--diff-steps", type=int, default=3,
+                    help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -188,6 +246,8 @@ def main():
                    per_patch_ms=cb["per_patch_s"] * 1e3, blas_threads=cb["blas"],
                    host_cpus=os.cpu_count())
 
+    diff = bench_differentiable(args.diff_steps) if args.diff_steps > 0 else None
+
     line = {
         "metric": "MP-PCA voxel-frames/s at 80x80x48x300",
         "value": value,
@@ -208,6 +268,7 @@ def main():
         "cpu_baseline": cpu,
         "stages": stages,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "differentiable": diff,
         "datagen_s": gen_s,
     }
     print(json.dumps(line))

@@ -110,6 +110,12 @@ typedef struct pdamd_problem {
   /* out: number of selected patches */
   int64_t selected;
 
+  /* OPTIMAL: per-patch threshold scale lambda (float64, indexed by linear
+     patch index; NULL = 1).  The noise scale c of every patch becomes
+     lambda_p * c, so lambda = 1 is the reference's rule.  No counterpart in
+     the reference (added for the differentiable path).                     */
+  const double* lam_dev;
+
   /* optional stage timing (host arrays of PDAMD_NSTAGES, NULL to skip): when
      set, HIP events are recorded on `stream` around every kernel launch and,
      after the call completes, stage_ms[s] += summed elapsed milliseconds and
@@ -127,7 +133,10 @@ enum {
   PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate   */
   PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask         */
   PDAMD_STAGE_TRIDIAG = 5,  /* tridiag_kernel: Householder tridiagonalisation  */
-  PDAMD_NSTAGES = 6
+  PDAMD_STAGE_ADJ_CROSS = 6,    /* backward: Xc^T Gbar per patch                */
+  PDAMD_STAGE_ADJ_SPECTRAL = 7, /* backward: singular-vector adjoint            */
+  PDAMD_STAGE_ADJ_SYNTH = 8,    /* backward: dL/dX scatter-accumulate           */
+  PDAMD_NSTAGES = 9
 };
 
 /* Bytes of device workspace pdamd_denoise needs for this problem. */
@@ -143,6 +152,19 @@ int pdamd_denoise(pdamd_problem* prob, void* workspace_dev, size_t workspace_byt
    Uses prob's shape/patch/step/mask fields.                               */
 int pdamd_select_patches(const pdamd_problem* prob, int32_t* index_dev, int64_t* count,
                          void* workspace_dev, size_t workspace_bytes, void* stream);
+
+/* Backward of OPTIMAL shrinkage (weighted / average / center recombination,
+   real data): given dL/d(output) (grad_out_dev, input dtype) and the forward's
+   weights (weights_dev), accumulate dL/d(input) into grad_in_dev (input dtype,
+   zeroed by this call) and write dL/d(lambda) per linear patch index into
+   grad_lam_dev (float64, zeroed by this call; NULL to skip).  U and S are
+   recomputed; the singular-vector adjoint uses guarded
+   (g_i -/+ g_j) / (s_i -/+ s_j) terms.  No counterpart in the reference (its
+   path is not differentiable).                                             */
+int pdamd_backward_workspace_size(const pdamd_problem* prob, size_t* bytes);
+int pdamd_optimal_backward(pdamd_problem* prob, const void* grad_out_dev,
+                           const float* weights_dev, void* grad_in_dev, double* grad_lam_dev,
+                           void* workspace_dev, size_t workspace_bytes, void* stream);
 
 const char* pdamd_last_error(void);
 const char* pdamd_version(void);

@@ -662,6 +662,7 @@ struct EigArgs {
   int scale_f64, tau_f64;
   double ts2, tau, mpm, beta;
   const double* var_map;
+  const double* LAM;  // optimal rule: per-linear-patch scale of the noise level c (null = 1)
   const void* H;
   const void* DE;
   const void* TAU;
@@ -884,6 +885,7 @@ __device__ double shrink(int loss, double y, double beta) {
 }
 
 constexpr int kKB = 16;  // components per eigenvector batch (registers per lane)
+constexpr int kAllPairs = 16;  // internal eigen-stage rule: every eigenpair, ascending
 
 // One wavefront per patch.  Eigenvalues by multisection on the tridiagonal,
 // the method's rule, then for the kept components: inverse iteration (one
@@ -1039,6 +1041,20 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       wave_sync();
       ncomp = sh_i[0];
       rank = sh_i[1];
+    } else if (a.method == kAllPairs) {
+      // every eigenpair, ascending (recomputation for the backward pass);
+      // WC holds the eigenvalues
+      for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
+      wave_sync();
+      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
+      for (int c = lane; c < n; c += kWave) {
+        cidx[c] = c;
+        lam[c] = s.val[c];
+        wc[c] = (double)s.val[c];
+      }
+      ncomp = n;
+      rank = 0;
+      wave_sync();
     } else {
       // ---- optimal shrinkage, lowrank.py:387-407
       double sigma;
@@ -1062,7 +1078,8 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         }
         sigma = (double)med / sqrt((double)a.g.pt * a.mpm);
       }
-      const double cs = sqrt((double)a.g.pt) * sigma;
+      const double cs =
+          sqrt((double)a.g.pt) * sigma * (a.LAM ? a.LAM[a.idx[p]] : 1.0);
       const double sb = sqrt(a.beta);
       // candidates above the upper bulk edge, and (Frobenius) below the lower one
       int ctop = n, cbot = 0;
@@ -1118,7 +1135,12 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
     R* Zp = reinterpret_cast<R*>(a.Z) + (size_t)p * n * a.kmax;
     R* scr = reinterpret_cast<R*>(a.SCR) + (size_t)p * 3 * n * KB;
     const R eps_piv = Lim<R>::ulp() * tnorm + pivmin;
-    const R clus = R(1e-3) * tnorm;
+    // Gram-Schmidt inside clusters of eigenvalues closer than clus: inverse
+    // iteration from eigenvalues accurate to ~ulp * tnorm yields vectors
+    // orthogonal to ~ulp * tnorm / gap, so FP64 needs it only for gaps below
+    // 1e-8 tnorm (error <= ~1e-8); FP32 keeps the classical 1e-3 tnorm.
+    const R clus = (sizeof(R) == 8 ? R(1e-8) : R(1e-3)) * tnorm;
+    const int n_init = a.method == kAllPairs ? 2 : 3;  // initial solves per vector
     const int i0 = lane, i1 = lane + kWave;  // rows held by this lane
     for (int c0 = 0; c0 < ncomp; c0 += KB) {
       const int kb = min(KB, ncomp - c0);
@@ -1137,7 +1159,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           h ^= h << 13; h ^= h >> 17; h ^= h << 5;
           zb[i * KB + lane] = R(0.5) + (R)(h & 0xFFFF) * R(1.0 / 65536.0);
         }
-        for (int it = 0; it < 3; ++it) solve(lane);
+        for (int it = 0; it < n_init; ++it) solve(lane);
       }
       wave_sync();
       // rows -> registers (lane l: rows l and l+64)
@@ -1292,7 +1314,7 @@ struct SynthArgs {
   const int* NCOMP;
   const double* THETA;
   void* out;
-  float* w;  // w3d ? voxel weights : element weights
+  double* w;  // w3d ? voxel weights : element weights (float64 accumulators)
 };
 
 __device__ __forceinline__ void atomic_add_(float* p, float v) { atomicAdd(p, v); }
@@ -1521,11 +1543,11 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
     if (!center) {
       if (a.w3d) {
         for (int r = tid; r < M; r += kNT)
-          atomicAdd(a.w + (sp + row_offset(g, r)) / g.T, (float)theta);
+          atomicAdd(a.w + (sp + row_offset(g, r)) / g.T, theta);
       } else {
         for (int q = tid; q < M * T; q += kNT) {
           const int r = q / T, t = q % T;
-          atomicAdd(a.w + sp + row_offset(g, r) + t0 + t, (float)theta);
+          atomicAdd(a.w + sp + row_offset(g, r) + t0 + t, theta);
         }
       }
     }
@@ -1556,23 +1578,26 @@ template <class R> __device__ __forceinline__ cpx<R> div_w(cpx<R> a, float w) {
   return {a.x / (R)w, a.y / (R)w};
 }
 
+// The weights are summed in float64 and rounded to float32 once here.  The
+// atomic summation order changes the float64 sum by ~1e-16 relative, which
+// survives the rounding only for sums that close to a float32 rounding
+// midpoint, so the float32 weights (and the normalised output) repeat from
+// call to call; atomic float32 sums would differ by an ulp between calls.
 template <class IN>
 __global__ void finalize_kernel(IN* __restrict__ out, float* __restrict__ w4,
-                                const float* __restrict__ w3, const uint8_t* __restrict__ mask,
-                                int mask_kind, int recomb, int64_t nvox, int64_t T) {
+                                const double* __restrict__ wacc, int w3d,
+                                const uint8_t* __restrict__ mask, int mask_kind, int recomb,
+                                int64_t nvox, int64_t T) {
   const int64_t total = nvox * T;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = q / T;
-    float wv;
-    if (w3) {
-      wv = w3[v];
-      if (recomb != PDAMD_CENTER) w4[q] = wv;
-    } else {
-      wv = w4[q];
-    }
     IN o = out[q];
-    if (recomb != PDAMD_CENTER) o = div_w(o, wv);
+    if (recomb != PDAMD_CENTER) {
+      const float wv = (float)wacc[w3d ? v : q];
+      w4[q] = wv;
+      o = div_w(o, wv);
+    }
     bool keep = true;
     if (mask_kind == PDAMD_MASK_SPATIAL) keep = mask[v] != 0;
     else if (mask_kind == PDAMD_MASK_FULL) keep = mask[q] != 0;
@@ -1656,9 +1681,13 @@ struct Plan {
   // workspace offsets
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
       off_WC, off_NC, off_TH, off_w3, off_SCR, total;
+  // backward only
+  size_t off_HC = 0, off_GS = 0, off_S1 = 0, off_S2 = 0, off_R1 = 0, off_R2 = 0, off_UV = 0,
+         off_THB = 0;
 };
 
-int make_plan(const pdamd_problem* P, Plan* pl) {
+// bwd: layout for pdamd_optimal_backward (all eigenpairs, adjoint buffers).
+int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   if (!P) return fail(PDAMD_EINVAL, "null problem");
   if (P->dtype < 0 || P->dtype > 3) return fail(PDAMD_EINVAL, "bad dtype %d", P->dtype);
   for (int i = 0; i < 4; ++i) {
@@ -1714,11 +1743,21 @@ int make_plan(const pdamd_problem* P, Plan* pl) {
   pl->rbytes = e64 ? 8 : 4;
   pl->ebytes = pl->rbytes * (cplx ? 2 : 1);
   pl->kmax = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? std::min(10, n) : n;
+  if (bwd) {
+    if (P->method != PDAMD_OPTIMAL)
+      return fail(PDAMD_EUNSUPPORTED, "the backward pass is built for the optimal rule only");
+    if (cplx) return fail(PDAMD_EUNSUPPORTED, "the backward pass is built for real data only");
+    if (!pl->tside)
+      return fail(PDAMD_EUNSUPPORTED,
+                  "the backward pass needs fewer frames than patch voxels (T-side Gram)");
+    pl->kmax = n;
+  }
   pl->kb = std::min(kKB, pl->kmax);
-  // chunk: keep the Householder block under ~1 GiB
-  const size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
-                     3 * n * pl->ebytes + pl->kmax * 8 + 16;
-  int64_t chunk = (int64_t)((1ull << 30) / per);
+  // chunk: keep the per-chunk buffers under ~1 GiB (4 GiB for the backward)
+  size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
+               3 * n * pl->ebytes + pl->kmax * 8 + 16;
+  if (bwd) per += 5 * (size_t)n * n * pl->ebytes + 2 * (size_t)n * pl->ebytes + 8;
+  int64_t chunk = (int64_t)((bwd ? (4ull << 30) : (1ull << 30)) / per);
   chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
   pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
   // workspace layout
@@ -1744,7 +1783,17 @@ int make_plan(const pdamd_problem* P, Plan* pl) {
   pl->off_WC = take(C * pl->kmax * sizeof(double));
   pl->off_NC = take(C * sizeof(int));
   pl->off_TH = take(C * sizeof(double));
-  pl->off_w3 = take((size_t)pl->nvox * sizeof(float));
+  pl->off_w3 = take((size_t)pl->nvox * (g.gt == 1 ? 1 : g.T) * sizeof(double));
+  if (bwd) {
+    pl->off_HC = take(C * n * n * pl->ebytes);
+    pl->off_GS = take(C * n * pl->ebytes);
+    pl->off_S1 = take(C * n * n * pl->ebytes);
+    pl->off_S2 = take(C * n * n * pl->ebytes);
+    pl->off_R1 = take(C * n * n * pl->ebytes);
+    pl->off_R2 = take(C * n * n * pl->ebytes);
+    pl->off_UV = take(C * n * pl->ebytes);
+    pl->off_THB = take(C * sizeof(double));
+  }
   pl->total = o;
   return PDAMD_OK;
 }
@@ -1855,23 +1904,64 @@ int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) 
   return PDAMD_OK;
 }
 
+EigArgs eig_args(const pdamd_problem* P, const Plan& pl, char* ws, const GramArgs& ga) {
+  EigArgs ea;
+  ea.g = pl.g;
+  ea.idx = ga.idx;
+  ea.np = ga.np;
+  ea.n = pl.n;
+  ea.kmax = pl.kmax;
+  ea.kb = pl.kb;
+  ea.method = P->method;
+  ea.loss = P->loss;
+  ea.rule_f64 = pl.rule_f64;
+  ea.scale_f64 = P->scale_f64;
+  ea.tau_f64 = P->tau_f64;
+  ea.ts2 = P->threshold_scale2;
+  ea.tau = P->tau;
+  ea.mpm = P->mp_median;
+  ea.beta = (double)pl.g.pt / (double)pl.g.M;
+  ea.var_map = P->var_map_dev;
+  ea.LAM = P->lam_dev;
+  ea.H = ga.H;
+  ea.DE = ga.DE;
+  ea.TAU = ga.TAU;
+  ea.Y = ws + pl.off_Y;
+  ea.Z = ws + pl.off_Z;
+  ea.SCR = ws + pl.off_SCR;
+  ea.WC = reinterpret_cast<double*>(ws + pl.off_WC);
+  ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_NC);
+  ea.THETA = reinterpret_cast<double*>(ws + pl.off_TH);
+  ea.RANK = nullptr;
+  ea.NOISE = nullptr;
+  ea.EIG = nullptr;
+  return ea;
+}
+
+GramArgs gram_args(const pdamd_problem* P, const Plan& pl, char* ws, const int32_t* idx, int np) {
+  GramArgs ga;
+  ga.g = pl.g;
+  ga.idx = idx;
+  ga.np = np;
+  ga.n = pl.n;
+  ga.scale = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? 1 : 0;
+  ga.H = ws + pl.off_H;
+  ga.DE = ws + pl.off_DE;
+  ga.TAU = ws + pl.off_TAU;
+  return ga;
+}
+
+#include "pd_adjoint.inl"
+
 template <class IN, class E, class RT>
 int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
                StageTimer& tm) {
   const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
-  float* w3 = reinterpret_cast<float*>(ws + pl.off_w3);
+  double* wacc = reinterpret_cast<double*>(ws + pl.off_w3);
   const bool w3d = pl.g.gt == 1;
   for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
     const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
-    GramArgs ga;
-    ga.g = pl.g;
-    ga.idx = idx_all + c0;
-    ga.np = np;
-    ga.n = pl.n;
-    ga.scale = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? 1 : 0;
-    ga.H = ws + pl.off_H;
-    ga.DE = ws + pl.off_DE;
-    ga.TAU = ws + pl.off_TAU;
+    GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
     tm.begin(PDAMD_STAGE_GRAM);
     int rc = pl.tside ? launch_gram_tile<IN, E, true>(pl, P->input_dev, ga, st)
                       : launch_gram_tile<IN, E, false>(pl, P->input_dev, ga, st);
@@ -1881,32 +1971,7 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     rc = launch_tridiag_tile<E>(pl, ga, st);
     tm.end();
     if (rc) return rc;
-    EigArgs ea;
-    ea.g = pl.g;
-    ea.idx = ga.idx;
-    ea.np = np;
-    ea.n = pl.n;
-    ea.kmax = pl.kmax;
-    ea.kb = pl.kb;
-    ea.method = P->method;
-    ea.loss = P->loss;
-    ea.rule_f64 = pl.rule_f64;
-    ea.scale_f64 = P->scale_f64;
-    ea.tau_f64 = P->tau_f64;
-    ea.ts2 = P->threshold_scale2;
-    ea.tau = P->tau;
-    ea.mpm = P->mp_median;
-    ea.beta = (double)pl.g.pt / (double)pl.g.M;
-    ea.var_map = P->var_map_dev;
-    ea.H = ga.H;
-    ea.DE = ga.DE;
-    ea.TAU = ga.TAU;
-    ea.Y = ws + pl.off_Y;
-    ea.Z = ws + pl.off_Z;
-    ea.SCR = ws + pl.off_SCR;
-    ea.WC = reinterpret_cast<double*>(ws + pl.off_WC);
-    ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_NC);
-    ea.THETA = reinterpret_cast<double*>(ws + pl.off_TH);
+    EigArgs ea = eig_args(P, pl, ws, ga);
     ea.RANK = P->rank_dev ? P->rank_dev + c0 : nullptr;
     ea.NOISE = P->noise_dev ? P->noise_dev + c0 : nullptr;
     ea.EIG = P->eig_dev ? P->eig_dev + c0 * 10 : nullptr;
@@ -1927,7 +1992,7 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     sa.NCOMP = ea.NCOMP;
     sa.THETA = ea.THETA;
     sa.out = P->output_dev;
-    sa.w = w3d ? w3 : P->weights_dev;
+    sa.w = wacc;
     tm.begin(PDAMD_STAGE_SYNTH);
     rc = pl.tside ? launch_synth<IN, E, true>(pl, P->input_dev, sa, st)
                   : launch_synth<IN, E, false>(pl, P->input_dev, sa, st);
@@ -1938,10 +2003,77 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
   const int nb = (int)std::min<int64_t>((total + 255) / 256, 65536);
   tm.begin(PDAMD_STAGE_FINALIZE);
   finalize_kernel<IN><<<nb, 256, 0, st>>>(reinterpret_cast<IN*>(P->output_dev), P->weights_dev,
-                                          w3d ? w3 : nullptr, P->mask_dev, P->mask_kind,
+                                          wacc, w3d ? 1 : 0, P->mask_dev, P->mask_kind,
                                           P->recombination, pl.nvox, pl.g.T);
   tm.end();
   HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class IN, class E, class RT>
+int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, const void* gout,
+                 const float* w4, void* gin, double* glam, hipStream_t st, StageTimer& tm) {
+  const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
+  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
+    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+    // recompute the Gram eigendecomposition (all pairs)
+    GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
+    tm.begin(PDAMD_STAGE_GRAM);
+    int rc = launch_gram_tile<IN, E, true>(pl, P->input_dev, ga, st);
+    tm.end();
+    if (rc) return rc;
+    tm.begin(PDAMD_STAGE_TRIDIAG);
+    rc = launch_tridiag_tile<E>(pl, ga, st);
+    tm.end();
+    if (rc) return rc;
+    EigArgs ea = eig_args(P, pl, ws, ga);
+    ea.method = kAllPairs;
+    ea.var_map = nullptr;
+    ea.LAM = nullptr;
+    tm.begin(PDAMD_STAGE_EIG);
+    rc = launch_eig<E, RT>(pl, ea, st);
+    tm.end();
+    if (rc) return rc;
+    AdjArgs aa;
+    aa.g = pl.g;
+    aa.idx = ga.idx;
+    aa.np = np;
+    aa.n = pl.n;
+    aa.recomb = P->recombination;
+    aa.mask_kind = P->mask_kind;
+    aa.mask = P->mask_dev;
+    aa.gout = gout;
+    aa.w4 = w4;
+    aa.HC = ws + pl.off_HC;
+    aa.GS = ws + pl.off_GS;
+    aa.Y = ea.Y;
+    aa.EV = ea.WC;
+    aa.S1 = ws + pl.off_S1;
+    aa.S2 = ws + pl.off_S2;
+    aa.R1 = ws + pl.off_R1;
+    aa.R2 = ws + pl.off_R2;
+    aa.UV = ws + pl.off_UV;
+    aa.THB = reinterpret_cast<double*>(ws + pl.off_THB);
+    aa.loss = P->loss;
+    aa.mpm = P->mp_median;
+    aa.beta = (double)pl.g.pt / (double)pl.g.M;
+    aa.var_map = P->var_map_dev;
+    aa.LAM = P->lam_dev;
+    aa.GLAM = glam;
+    aa.gin = gin;
+    tm.begin(PDAMD_STAGE_ADJ_CROSS);
+    rc = launch_adj_patch_kernel<IN, E>(pl, adj_cross_kernel<IN, E>, P->input_dev, aa, st);
+    tm.end();
+    if (rc) return rc;
+    tm.begin(PDAMD_STAGE_ADJ_SPECTRAL);
+    rc = launch_adj_spectral<E, RT>(pl, aa, st);
+    tm.end();
+    if (rc) return rc;
+    tm.begin(PDAMD_STAGE_ADJ_SYNTH);
+    rc = launch_adj_patch_kernel<IN, E>(pl, adj_synth_kernel<IN, E>, P->input_dev, aa, st);
+    tm.end();
+    if (rc) return rc;
+  }
   return PDAMD_OK;
 }
 
@@ -1981,12 +2113,16 @@ int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes,
     return fail(PDAMD_EINVAL, "null data pointer");
   if ((P->method == PDAMD_HYBRID) && !P->var_map_dev)
     return fail(PDAMD_EINVAL, "hybrid rule needs a variance map");
+  if (P->lam_dev && P->method != PDAMD_OPTIMAL)
+    return fail(PDAMD_EINVAL, "a threshold scale map is only used by the optimal rule");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   char* ws = reinterpret_cast<char*>(workspace_dev);
   const size_t esz = (P->dtype == PDAMD_F32) ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
   HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
-  HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
-  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0, (size_t)pl.nvox * sizeof(float), st));
+  if (P->recombination == PDAMD_CENTER)  // otherwise the normalisation writes every weight
+    HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
+  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0,
+                         (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st));
   int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
   int64_t count = 0;
   StageTimer tm;
@@ -2013,6 +2149,61 @@ int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes,
       break;
     default:
       return fail(PDAMD_EUNSUPPORTED, "dtype/precision combination not built");
+  }
+  if (rc) return rc;
+  if (tm.collect(P->stage_ms, P->stage_launches))
+    return fail(PDAMD_EHIP, "stream failed while collecting stage timings");
+  return PDAMD_OK;
+}
+
+int pdamd_backward_workspace_size(const pdamd_problem* prob, size_t* bytes) {
+  Plan pl;
+  const int rc = make_plan(prob, &pl, true);
+  if (rc) return rc;
+  *bytes = pl.total;
+  return PDAMD_OK;
+}
+
+int pdamd_optimal_backward(pdamd_problem* P, const void* grad_out_dev, const float* weights_dev,
+                           void* grad_in_dev, double* grad_lam_dev, void* workspace_dev,
+                           size_t workspace_bytes, void* stream) {
+  Plan pl;
+  int rc = make_plan(P, &pl, true);
+  if (rc) return rc;
+  if (workspace_bytes < pl.total)
+    return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, pl.total);
+  if (!P->input_dev || !grad_out_dev || !grad_in_dev)
+    return fail(PDAMD_EINVAL, "null data pointer");
+  if (!weights_dev && P->recombination != PDAMD_CENTER)
+    return fail(PDAMD_EINVAL, "the forward weights are needed for this recombination");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* ws = reinterpret_cast<char*>(workspace_dev);
+  const size_t esz = (P->dtype == PDAMD_F32) ? 4 : 8;
+  HIP_TRY(hipMemsetAsync(grad_in_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
+  if (grad_lam_dev) HIP_TRY(hipMemsetAsync(grad_lam_dev, 0, (size_t)pl.npatch * sizeof(double), st));
+  int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
+  int64_t count = 0;
+  StageTimer tm;
+  tm.on = P->stage_ms != nullptr || P->stage_launches != nullptr;
+  tm.st = st;
+  rc = run_select(P, pl, ws, idx, &count, st, &tm);
+  if (rc) return rc;
+  P->selected = count;
+  switch (P->dtype * 4 + pl.etype) {
+    case PDAMD_F32 * 4 + 0:
+      rc = run_backward<float, float, float>(P, pl, ws, count, grad_out_dev, weights_dev,
+                                             grad_in_dev, grad_lam_dev, st, tm);
+      break;
+    case PDAMD_F32 * 4 + 1:
+      rc = run_backward<float, double, float>(P, pl, ws, count, grad_out_dev, weights_dev,
+                                              grad_in_dev, grad_lam_dev, st, tm);
+      break;
+    case PDAMD_F64 * 4 + 1:
+      rc = run_backward<double, double, double>(P, pl, ws, count, grad_out_dev, weights_dev,
+                                                grad_in_dev, grad_lam_dev, st, tm);
+      break;
+    default:
+      return fail(PDAMD_EUNSUPPORTED, "dtype/precision combination not built for the backward");
   }
   if (rc) return rc;
   if (tm.collect(P->stage_ms, P->stage_launches))

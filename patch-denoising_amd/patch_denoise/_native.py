@@ -59,18 +59,22 @@ class Problem(ctypes.Structure):
         ("noise_dev", ctypes.c_void_p),
         ("eig_dev", ctypes.c_void_p),
         ("selected", ctypes.c_int64),
+        ("lam_dev", ctypes.c_void_p),
         ("stage_ms", ctypes.c_void_p),
         ("stage_launches", ctypes.c_void_p),
     ]
 
 
-STAGES = ("select", "gram", "trieig", "synth", "finalize", "tridiag")
+STAGES = ("select", "gram", "trieig", "synth", "finalize", "tridiag", "adj_cross", "adj_spectral",
+          "adj_synth")
 
 
 EXPORTS = (
     "pdamd_workspace_size",
     "pdamd_denoise",
     "pdamd_select_patches",
+    "pdamd_backward_workspace_size",
+    "pdamd_optimal_backward",
     "pdamd_last_error",
     "pdamd_version",
 )
@@ -99,6 +103,13 @@ def load_library():
         ctypes.c_void_p,
     ]
     lib.pdamd_select_patches.restype = ctypes.c_int
+    lib.pdamd_backward_workspace_size.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
+    lib.pdamd_backward_workspace_size.restype = ctypes.c_int
+    lib.pdamd_optimal_backward.argtypes = [
+        P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_size_t, ctypes.c_void_p,
+    ]
+    lib.pdamd_optimal_backward.restype = ctypes.c_int
     lib.pdamd_last_error.restype = ctypes.c_char_p
     lib.pdamd_version.restype = ctypes.c_char_p
     _lib = lib
