@@ -13,6 +13,8 @@
 //             |s_i - s_j| <= 1e-8 max(s); Gp_ii = f'(s_i / c)),
 //     Gm_ij = (g_i + g_j) / (s_i + s_j),
 //   dL/dXc = Xc R1 + Gbar R2,  R1 = V S^-1 (Q - C diag(h)) V^T,  R2 = V diag(h) V^T,
+//          = (P1 D + P2 diag(h)) V^T  with P1 = Xc V, P2 = Gbar V, D = S^-1 (Q - C diag(h)),
+//   and C = S^-1 P1^T P2 (so H is never formed),
 //   cbar = sum_i C_ii (f(y_i) - y_i f'(y_i)),  y = s / c,
 //   dL/dlambda = cbar * c0, and when c0 comes from the median singular value
 //   Q_mm += cbar * lambda / sqrt(mp_median) at the median index (half to each
@@ -72,63 +74,143 @@ __device__ void shrink_fd(int loss, double y, double beta, double* f, double* fp
 // ---------------------------------------------------------------------------
 // Block GEMM for one patch: C (m x nn, nn <= 128) = A (m x k) B (k x nn),
 // 256 threads, each owning an 8 x 8 register tile of a 128 x 128 block;
-// k streamed in panels of kGP through LDS.  Operands come from functors
-// la(i, kk) / lb(kk, j); AKF / BKF say whether consecutive k of an operand are
-// contiguous in memory (sets the staging order for coalesced loads).
-// st(i, j, value) consumes the result.
+// k streamed in panels of kGP through double-buffered LDS: the next panel's
+// elements are loaded into registers while the current one is multiplied,
+// one barrier per panel.  Operands come from functors la(i, kk) / lb(kk, j);
+// AKF / BKF say whether consecutive k of an operand are contiguous in memory
+// (sets the staging order for coalesced loads).  st(i, j, value) consumes
+// the result.
 // ---------------------------------------------------------------------------
 constexpr int kGP = 8;
 constexpr int kGLD = 128 + 4;
+typedef double double4_t __attribute__((ext_vector_type(4)));
+constexpr int kGSE = kGP * 128 / kNT;  // staged elements per thread and operand
 
 template <class R>
 size_t gemm_lds_bytes() {
-  return 2 * (size_t)kGP * kGLD * sizeof(R);
+  return 4 * (size_t)kGP * kGLD * sizeof(R);  // A and B, two buffers each
 }
 
 template <class R, bool AKF, bool BKF, class LA, class LB, class ST>
 __device__ __forceinline__ void tile_gemm(int m, int nn, int k, LA la, LB lb, ST st, R* As,
                                           R* Bs) {
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  constexpr int BUF = kGP * kGLD;
+  // FP64 on the matrix cores (v_mfma_f64_16x16x4f64), FP32 on the VALU
+  constexpr bool MF = std::is_same<R, double>::value;
+  const int wv = tid / kWave, lane = tid % kWave;
+  const int wr = wv >> 1, wc = wv & 1, lhi = lane >> 4, llo = lane & 15;
   for (int m0 = 0; m0 < m; m0 += 128) {
     const int mb = min(128, m - m0);
-    R acc[8][8];
+    R acc[MF ? 1 : 8][MF ? 1 : 8];
+    double4_t macc[4][4];
+    if constexpr (MF) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+      for (int ta = 0; ta < 4; ++ta)
 #pragma unroll
-      for (int v = 0; v < 8; ++v) acc[u][v] = R(0);
-    for (int k0 = 0; k0 < k; k0 += kGP) {
-      const int kc = min(kGP, k - k0);
-      __syncthreads();
-      for (int e = tid; e < kGP * 128; e += kNT) {
-        const int kk = AKF ? e % kGP : e / 128, i = AKF ? e / kGP : e % 128;
-        As[kk * kGLD + i] = (i < mb && kk < kc) ? la(m0 + i, k0 + kk) : R(0);
-      }
-      for (int e = tid; e < kGP * 128; e += kNT) {
-        const int kk = BKF ? e % kGP : e / 128, j = BKF ? e / kGP : e % 128;
-        Bs[kk * kGLD + j] = (j < nn && kk < kc) ? lb(k0 + kk, j) : R(0);
-      }
-      __syncthreads();
+        for (int tb = 0; tb < 4; ++tb) macc[ta][tb] = double4_t{0.0, 0.0, 0.0, 0.0};
+    } else {
 #pragma unroll
-      for (int kk = 0; kk < kGP; ++kk) {
-        R a[8], b[8];
+      for (int u = 0; u < 8; ++u)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          a[u] = As[kk * kGLD + tr * 8 + u];
-          b[u] = Bs[kk * kGLD + tc * 8 + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-#pragma unroll
-          for (int v = 0; v < 8; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
-      }
+        for (int v = 0; v < 8; ++v) acc[u][v] = R(0);
     }
+    R ra[kGSE], rb[kGSE];
+    auto fetch = [&](int k0) {
+      const int kc = min(kGP, k - k0);
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int i = tr * 8 + u, j = tc * 8 + v;
-        if (i < mb && j < nn) st(m0 + i, j, acc[u][v]);
+      for (int u = 0; u < kGSE; ++u) {
+        const int e = tid + u * kNT;
+        const int ka = AKF ? e % kGP : e / 128, ia = AKF ? e / kGP : e % 128;
+        ra[u] = (ia < mb && ka < kc) ? la(m0 + ia, k0 + ka) : R(0);
+        const int kb = BKF ? e % kGP : e / 128, jb = BKF ? e / kGP : e % 128;
+        rb[u] = (jb < nn && kb < kc) ? lb(k0 + kb, jb) : R(0);
       }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+      for (int u = 0; u < kGSE; ++u) {
+        const int e = tid + u * kNT;
+        const int ka = AKF ? e % kGP : e / 128, ia = AKF ? e / kGP : e % 128;
+        As[buf * BUF + ka * kGLD + ia] = ra[u];
+        const int kb = BKF ? e % kGP : e / 128, jb = BKF ? e / kGP : e % 128;
+        Bs[buf * BUF + kb * kGLD + jb] = rb[u];
+      }
+    };
+    __syncthreads();  // previous users of the buffers are done
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < k; k0 += kGP) {
+      const bool more = k0 + kGP < k;
+      if (more) fetch(k0 + kGP);
+      const R* Ab = As + buf * BUF;
+      const R* Bb = Bs + buf * BUF;
+      if constexpr (MF) {
+        // wave (wr, wc) owns rows 64 wr .. + 64 and columns 64 wc .. + 64:
+        // 4 x 4 MFMA tiles of 16 x 16, k in steps of 4.  Lane l supplies
+        // A[16 t + l % 16][k + l / 16] and B[k + l / 16][16 t + l % 16].
+#pragma unroll
+        for (int kq = 0; kq < kGP; kq += 4) {
+          R fa[4], fb[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            fa[t] = Ab[(kq + lhi) * kGLD + 64 * wr + 16 * t + llo];
+            fb[t] = Bb[(kq + lhi) * kGLD + 64 * wc + 16 * t + llo];
+          }
+#pragma unroll
+          for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb)
+              macc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ta], fb[tb], macc[ta][tb],
+                                                                  0, 0, 0);
+        }
+      } else {
+#pragma unroll 2
+        for (int kk = 0; kk < kGP; ++kk) {
+          R a[8], b[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a[u] = Ab[kk * kGLD + tr * 8 + u];
+            b[u] = Bb[kk * kGLD + tc * 8 + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
+      }
+      if (more) stash(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+    if constexpr (MF) {
+      // D[16 ta + 4 q + l / 16][16 tb + l % 16] = macc[ta][tb][q]
+#pragma unroll
+      for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+          {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int i = 64 * wr + 16 * ta + 4 * q + lhi, j = 64 * wc + 16 * tb + llo;
+              if (i < mb && j < nn) st(m0 + i, j, macc[ta][tb][q]);
+            }
+            // one 16 x 16 tile at a time: keeps the epilogue's operand loads
+            // from being hoisted for all 64 elements at once
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int i = tr * 8 + u, j = tc * 8 + v;
+          if (i < mb && j < nn) st(m0 + i, j, acc[u][v]);
+        }
+    }
   }
   __syncthreads();
 }
@@ -142,41 +224,58 @@ struct AdjArgs {
   const uint8_t* mask;
   const void* gout;     // dL/d(output), input dtype, nvox x T
   const float* w4;      // forward weights, nvox x T
-  void* HC;             // np x n x n   Xc^T Ghat
-  void* GS;             // np x n       column sums of Ghat
+  void* GH;             // nvox x T  Ghat = dL/d(output) / weights (masked), element type
   const void* Y;        // np x n x n   Gram eigenvectors V[i][c], ascending eigenvalues
   const double* EV;     // np x n       Gram eigenvalues (= s^2), ascending
-  void* S1;             // np x n x n   scratch
-  void* S2;             // np x n x n   scratch
-  void* R1;             // np x n x n
-  void* R2;             // np x n x n   theta * V diag(h) V^T
+  void* P1;             // np x M x n   Xc V
+  void* P2;             // np x M x n   Ghat V
+  void* PE;             // np x M x n   P1 D + theta P2 diag(h)
+  void* GS;             // np x n       column sums of Ghat
+  void* S1;             // np x n x n   P1^T P2, then D
   void* UV;             // np x n       u
   double* THB;          // np           theta
   int loss;
   double mpm, beta;
   const double* var_map;
-  const double* LAM;     // per linear patch index, or null (= 1)
-  double* GLAM;          // per linear patch index, or null
+  const double* LAM;    // per linear patch index, or null (= 1)
+  double* GLAM;         // per linear patch index, or null
   void* gin;            // dL/d(input), input dtype, nvox x T
 };
 
-// Ghat = dL/dP / theta at volume element q (voxel-major element index): the
-// output gradient divided by the forward weights (weighted / average) or the
-// gradient itself at the patch centre (center); zero where the output was
-// masked or never written.
+// Ghat = dL/dP / theta over the whole volume: the output gradient divided by
+// the forward weights (weighted / average) or the gradient itself (center,
+// where only patch centres read it); zero where the output was masked or
+// never written.  One elementwise pass per backward call.
 template <class IN, class E>
-__device__ __forceinline__ E ghat_at(const AdjArgs& a, int64_t q, bool centre_elem) {
+__global__ void ghat_kernel(AdjArgs a, int64_t total) {
   using R = typename real_of<E>::type;
   const IN* go = reinterpret_cast<const IN*>(a.gout);
-  const int64_t v = q / a.g.T;
-  bool keep = true;
-  if (a.mask_kind == PDAMD_MASK_SPATIAL) keep = a.mask[v] != 0;
-  else if (a.mask_kind == PDAMD_MASK_FULL) keep = a.mask[q] != 0;
-  if (!keep) return E(R(0));
-  if (a.recomb == PDAMD_CENTER) return centre_elem ? cvt<E>(go[q]) : E(R(0));
-  const float w = a.w4[q];
-  if (!(w > 0.f)) return E(R(0));
-  return cvt<E>(go[q]) / E(R(w));
+  E* gh = reinterpret_cast<E*>(a.GH);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = q / a.g.T;
+    bool keep = true;
+    if (a.mask_kind == PDAMD_MASK_SPATIAL) keep = a.mask[v] != 0;
+    else if (a.mask_kind == PDAMD_MASK_FULL) keep = a.mask[q] != 0;
+    E r = E(R(0));
+    if (keep) {
+      if (a.recomb == PDAMD_CENTER) {
+        r = cvt<E>(go[q]);
+      } else {
+        const float w = a.w4[q];
+        if (w > 0.f) r = cvt<E>(go[q]) / E(R(w));
+      }
+    }
+    gh[q] = r;
+  }
+}
+
+// Ghat element of a patch (centre recombination: only the patch centre).
+template <class E>
+__device__ __forceinline__ E ghat_at(const AdjArgs& a, int64_t q, bool centre_elem) {
+  using R = typename real_of<E>::type;
+  if (a.recomb == PDAMD_CENTER && !centre_elem) return E(R(0));
+  return reinterpret_cast<const E*>(a.GH)[q];
 }
 
 // Stage the patch's row offsets and NumPy-order per-frame mean in LDS.
@@ -186,11 +285,8 @@ __device__ __forceinline__ void stage_rows_mean(const Geom& g, const IN* X, int6
   const int M = g.M, T = g.pt;
   for (int r = threadIdx.x; r < M; r += kNT) roff[r] = row_offset(g, r);
   __syncthreads();
-  for (int t = threadIdx.x; t < T; t += kNT) {
-    IN s = X[roff[0] + t];
-    for (int r = 1; r < M; ++r) s += X[roff[r] + t];  // sequential, as numpy's axis-0 sum
-    mean[t] = div_n(s, M);
-  }
+  for (int t = threadIdx.x; t < T; t += kNT)  // sequential, as numpy's axis-0 sum
+    mean[t] = div_n(seq_add(X[roff[0] + t], X + t, roff + 1, M - 1), M);
   __syncthreads();
 }
 
@@ -201,16 +297,16 @@ size_t adj_patch_lds_bytes(int M, int T) {
   return gemm_lds_bytes<R>() + al((size_t)M * sizeof(int64_t)) + al((size_t)T * sizeof(IN)) + 16;
 }
 
-// H = Xc^T Ghat (n x n) and the column sums of Ghat, per patch.
+// P1 = Xc V, P2 = Ghat V (M x n each) and the column sums of Ghat, per patch.
 template <class IN, class E>
-__global__ __launch_bounds__(kNT) void adj_cross_kernel(const IN* __restrict__ vol, AdjArgs a) {
+__global__ __launch_bounds__(kNT) void adj_project_kernel(const IN* __restrict__ vol, AdjArgs a) {
   using R = typename real_of<E>::type;
   const Geom& g = a.g;
   const int n = a.n, M = g.M, T = g.pt;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  R* As = lds_carve<R>(cur, (size_t)kGP * kGLD);
-  R* Bs = lds_carve<R>(cur, (size_t)kGP * kGLD);
+  R* As = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
+  R* Bs = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
   int64_t* roff = lds_carve<int64_t>(cur, M);
   IN* mean = lds_carve<IN>(cur, T);
   const int rc = ((g.p0 / 2) * g.p1 + g.p1 / 2) * g.p2 + g.p2 / 2, tcen = g.pt / 2;
@@ -221,38 +317,53 @@ __global__ __launch_bounds__(kNT) void adj_cross_kernel(const IN* __restrict__ v
     const IN* X = vol + sp + t0;
     stage_rows_mean(g, X, roff, mean);
     const int64_t base = sp + t0;
-    R* H = reinterpret_cast<R*>(a.HC) + (size_t)p * n * n;
+    const R* V = reinterpret_cast<const R*>(a.Y) + (size_t)p * n * n;
+    R* P1 = reinterpret_cast<R*>(a.P1) + (size_t)p * M * n;
+    R* P2 = reinterpret_cast<R*>(a.P2) + (size_t)p * M * n;
     R* GS = reinterpret_cast<R*>(a.GS) + (size_t)p * n;
-    tile_gemm<R, false, false>(
-        n, n, M, [&](int i, int kk) { return cvt<E>(X[roff[kk] + i] - mean[i]); },
-        [&](int kk, int j) {
-          return ghat_at<IN, E>(a, base + roff[kk] + j, kk == rc && j == tcen);
-        },
-        [&](int i, int j, R v) { H[(size_t)i * n + j] = v; }, As, Bs);
+    tile_gemm<R, true, false>(
+        M, n, n, [&](int i, int kk) { return cvt<E>(X[roff[i] + kk] - mean[kk]); },
+        [&](int kk, int j) { return V[(size_t)kk * n + j]; },
+        [&](int i, int j, R v) { P1[(size_t)i * n + j] = v; }, As, Bs);
+    tile_gemm<R, true, false>(
+        M, n, n,
+        [&](int i, int kk) { return ghat_at<E>(a, base + roff[i] + kk, i == rc && kk == tcen); },
+        [&](int kk, int j) { return V[(size_t)kk * n + j]; },
+        [&](int i, int j, R v) { P2[(size_t)i * n + j] = v; }, As, Bs);
     for (int j = threadIdx.x; j < n; j += kNT) {
       R s = R(0);
-      for (int r = 0; r < M; ++r) s += ghat_at<IN, E>(a, base + roff[r] + j, r == rc && j == tcen);
+      int r = 0;
+      for (; r + 8 <= M; r += 8) {
+        R gg[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) gg[u] = ghat_at<E>(a, base + roff[r + u] + j, r + u == rc && j == tcen);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += gg[u];
+      }
+      for (; r < M; ++r) s += ghat_at<E>(a, base + roff[r] + j, r == rc && j == tcen);
       GS[j] = s;
     }
     __syncthreads();
   }
 }
 
-// Spectral part of the adjoint, per patch: theta, dL/dlambda, R1, R2, u.
+// Spectral part of the adjoint, per patch: theta, dL/dlambda, C, D, u and
+// PE = P1 D + theta P2 diag(h).
 template <class E, class RT>
 __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
   using R = typename real_of<E>::type;
   const Geom& g = a.g;
-  const int n = a.n, tid = threadIdx.x;
+  const int n = a.n, M = g.M, tid = threadIdx.x;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  R* As = lds_carve<R>(cur, (size_t)kGP * kGLD);
-  R* Bs = lds_carve<R>(cur, (size_t)kGP * kGLD);
+  R* As = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
+  R* Bs = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
   double* sg = lds_carve<double>(cur, n);   // s_i
   double* gv = lds_carve<double>(cur, n);   // g_i
   double* fp = lds_carve<double>(cur, n);   // f'(y_i) = dg_i / ds_i
   double* dc = lds_carve<double>(cur, n);   // dg_i / dc
   double* hv = lds_carve<double>(cur, n);   // g_i / s_i
+  double* tv = lds_carve<double>(cur, n);   // h o (V^T gs)
   double* red = lds_carve<double>(cur, 8);
   __shared__ int sh_min;
   __shared__ double sh_s[4];
@@ -261,12 +372,11 @@ __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
     const int64_t L = a.idx[p];
     const double* ev = a.EV + (size_t)p * n;
     const R* V = reinterpret_cast<const R*>(a.Y) + (size_t)p * n * n;
-    const R* H = reinterpret_cast<const R*>(a.HC) + (size_t)p * n * n;
+    const R* P1 = reinterpret_cast<const R*>(a.P1) + (size_t)p * M * n;
+    const R* P2 = reinterpret_cast<const R*>(a.P2) + (size_t)p * M * n;
+    R* PE = reinterpret_cast<R*>(a.PE) + (size_t)p * M * n;
     const R* GS = reinterpret_cast<const R*>(a.GS) + (size_t)p * n;
     R* S1 = reinterpret_cast<R*>(a.S1) + (size_t)p * n * n;
-    R* S2 = reinterpret_cast<R*>(a.S2) + (size_t)p * n * n;
-    R* R1 = reinterpret_cast<R*>(a.R1) + (size_t)p * n * n;
-    R* R2 = reinterpret_cast<R*>(a.R2) + (size_t)p * n * n;
     R* UV = reinterpret_cast<R*>(a.UV) + (size_t)p * n;
     // ---- noise scale c (same arithmetic as the forward rule)
     if (tid < kWave) {
@@ -320,21 +430,30 @@ __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
     const int rank = n - sh_min;
     const double theta = a.recomb == PDAMD_WEIGHTED ? 1.0 / (2.0 + (double)rank) : 1.0;
     const double smax = sg[n - 1];
-    // ---- W = V^T (H V)
-    tile_gemm<R, true, false>(
-        n, n, n, [&](int i, int kk) { return H[(size_t)i * n + kk]; },
-        [&](int kk, int j) { return V[(size_t)kk * n + j]; },
-        [&](int i, int j, R v) { S1[(size_t)i * n + j] = v; }, As, Bs);
+    // ---- W = P1^T P2 (= V^T Xc^T Ghat V) into S1
     tile_gemm<R, false, false>(
-        n, n, n, [&](int i, int kk) { return V[(size_t)kk * n + i]; },
-        [&](int kk, int j) { return S1[(size_t)kk * n + j]; },
-        [&](int i, int j, R v) { S2[(size_t)i * n + j] = v; }, As, Bs);
-    // ---- cbar = sum_i C_ii dg_i/dc, C = theta S^-1 W
+        n, n, M, [&](int i, int kk) { return P1[(size_t)kk * n + i]; },
+        [&](int kk, int j) { return P2[(size_t)kk * n + j]; },
+        [&](int i, int j, R v) { S1[(size_t)i * n + j] = v; }, As, Bs);
+    // ---- cbar = sum_i C_ii dg_i/dc, C = theta S^-1 W;  t = h o (V^T gs)
     double part = 0.0;
     for (int i = tid; i < n; i += kNT)
-      if (sg[i] > 0.0) part += theta * (double)S2[(size_t)i * n + i] / sg[i] * dc[i];
+      if (sg[i] > 0.0) part += theta * (double)S1[(size_t)i * n + i] / sg[i] * dc[i];
     part = dpp_sum(part);
     if ((tid % kWave) == 0) red[tid / kWave] = part;
+    for (int c = tid; c < n; c += kNT) {
+      R t = R(0);
+      int r = 0;
+      for (; r + 8 <= n; r += 8) {
+        R vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vv[u] = V[(size_t)(r + u) * n + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t += vv[u] * GS[r + u];
+      }
+      for (; r < n; ++r) t += V[(size_t)r * n + c] * GS[r];
+      tv[c] = hv[c] * (double)t;
+    }
     __syncthreads();
     const double cbar = (red[0] + red[1]) + (red[2] + red[3]);
     double dmed = 0.0;  // extra Q on the median diagonal(s)
@@ -349,43 +468,52 @@ __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
         dmed *= 0.5;
       }
     }
-    // ---- D = S^-1 (Q - C diag(h)) into S1
+    // ---- D = S^-1 (Q - C diag(h)) in place of W: one thread per pair {i, j},
+    // which reads W_ij, W_ji and writes D_ij, D_ji (the divided differences
+    // of g are symmetric in i, j and computed once per pair)
     const double tie = 1e-8 * smax;
     for (int e = tid; e < n * n; e += kNT) {
       const int i = e / n, j = e % n;
+      if (i > j) continue;
       const double si = sg[i], sj = sg[j];
-      const double ci = si > 0.0 ? theta * (double)S2[(size_t)i * n + j] / si : 0.0;
-      const double cj = sj > 0.0 ? theta * (double)S2[(size_t)j * n + i] / sj : 0.0;
-      double q;
+      const double ii = si > 0.0 ? 1.0 / si : 0.0;  // zero rows for s = 0
+      const double ij = sj > 0.0 ? 1.0 / sj : 0.0;
+      const double cij = theta * (double)S1[(size_t)i * n + j] * ii;
+      const double cji = theta * (double)S1[(size_t)j * n + i] * ij;
       if (i == j) {
-        q = fp[i] * ci;
+        double q = fp[i] * cij;
         if (i == m0 || i == m1) q += dmed;
-      } else {
-        const double gp = fabs(si - sj) > tie ? (gv[i] - gv[j]) / (si - sj) : 0.5 * (fp[i] + fp[j]);
-        const double gm = (si + sj) > 0.0 ? (gv[i] + gv[j]) / (si + sj) : 0.0;
-        q = gp * 0.5 * (ci + cj) + gm * 0.5 * (ci - cj);
+        S1[(size_t)i * n + i] = (R)((q - cij * hv[i]) * ii);
+        continue;
       }
-      S1[(size_t)i * n + j] = si > 0.0 ? (R)((q - ci * hv[j]) / si) : R(0);
+      const double gp = fabs(si - sj) > tie ? (gv[i] - gv[j]) / (si - sj) : 0.5 * (fp[i] + fp[j]);
+      const double gm = (si + sj) > 0.0 ? (gv[i] + gv[j]) / (si + sj) : 0.0;
+      const double sym = 0.5 * gp * (cij + cji), skw = 0.5 * gm * (cij - cji);
+      S1[(size_t)i * n + j] = (R)((sym + skw - cij * hv[j]) * ii);
+      S1[(size_t)j * n + i] = (R)((sym - skw - cji * hv[i]) * ij);
     }
     __syncthreads();
-    // ---- R1 = V D V^T, R2 = theta V diag(h) V^T
+    // ---- PE = P1 D + theta P2 diag(h)
     tile_gemm<R, true, false>(
-        n, n, n, [&](int i, int kk) { return V[(size_t)i * n + kk]; },
+        M, n, n, [&](int i, int kk) { return P1[(size_t)i * n + kk]; },
         [&](int kk, int j) { return S1[(size_t)kk * n + j]; },
-        [&](int i, int j, R v) { S2[(size_t)i * n + j] = v; }, As, Bs);
-    tile_gemm<R, true, true>(
-        n, n, n, [&](int i, int kk) { return S2[(size_t)i * n + kk]; },
-        [&](int kk, int j) { return V[(size_t)j * n + kk]; },
-        [&](int i, int j, R v) { R1[(size_t)i * n + j] = v; }, As, Bs);
-    tile_gemm<R, true, true>(
-        n, n, n, [&](int i, int kk) { return (R)(theta * hv[kk]) * V[(size_t)i * n + kk]; },
-        [&](int kk, int j) { return V[(size_t)j * n + kk]; },
-        [&](int i, int j, R v) { R2[(size_t)i * n + j] = v; }, As, Bs);
-    // ---- u = theta * gs - R2 gs
+        [&](int i, int j, R v) {
+          PE[(size_t)i * n + j] = v + (R)(theta * hv[j]) * P2[(size_t)i * n + j];
+        },
+        As, Bs);
+    // ---- u = theta (gs - V (h o V^T gs))
     for (int j = tid; j < n; j += kNT) {
       R s = R(0);
-      for (int t = 0; t < n; ++t) s += R2[(size_t)j * n + t] * GS[t];
-      UV[j] = (R)theta * GS[j] - s;
+      int c = 0;
+      for (; c + 8 <= n; c += 8) {
+        R vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vv[u] = V[(size_t)j * n + c + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += vv[u] * (R)tv[c + u];
+      }
+      for (; c < n; ++c) s += V[(size_t)j * n + c] * (R)tv[c];
+      UV[j] = (R)theta * (GS[j] - s);
     }
     if (tid == 0) {
       a.THB[p] = theta;
@@ -395,44 +523,108 @@ __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
   }
 }
 
-// dL/dX = Xc R1 + Ghat R2 + 1 u^T / M, scatter-added into dL/d(input).
+// dL/dX = PE V^T + 1 u^T / M, scatter-added into dL/d(input).
 template <class IN, class E>
 __global__ __launch_bounds__(kNT) void adj_synth_kernel(const IN* __restrict__ vol, AdjArgs a) {
   using R = typename real_of<E>::type;
   const Geom& g = a.g;
-  const int n = a.n, M = g.M, T = g.pt;
+  const int n = a.n, M = g.M;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  R* As = lds_carve<R>(cur, (size_t)kGP * kGLD);
-  R* Bs = lds_carve<R>(cur, (size_t)kGP * kGLD);
+  R* As = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
+  R* Bs = lds_carve<R>(cur, (size_t)2 * kGP * kGLD);
   int64_t* roff = lds_carve<int64_t>(cur, M);
-  IN* mean = lds_carve<IN>(cur, T);
-  const int rc = ((g.p0 / 2) * g.p1 + g.p1 / 2) * g.p2 + g.p2 / 2, tcen = g.pt / 2;
   IN* gin = reinterpret_cast<IN*>(a.gin);
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
-    const IN* X = vol + sp + t0;
-    stage_rows_mean(g, X, roff, mean);
+    for (int r = threadIdx.x; r < M; r += kNT) roff[r] = row_offset(g, r);
+    __syncthreads();
     const int64_t base = sp + t0;
-    const R* R1 = reinterpret_cast<const R*>(a.R1) + (size_t)p * n * n;
-    const R* R2 = reinterpret_cast<const R*>(a.R2) + (size_t)p * n * n;
+    const R* V = reinterpret_cast<const R*>(a.Y) + (size_t)p * n * n;
+    const R* PE = reinterpret_cast<const R*>(a.PE) + (size_t)p * M * n;
     const R* UV = reinterpret_cast<const R*>(a.UV) + (size_t)p * n;
     const R invM = R(1) / (R)M;
-    tile_gemm<R, true, false>(
-        M, n, 2 * n,
-        [&](int i, int kk) -> R {
-          if (kk < n) return cvt<E>(X[roff[i] + kk] - mean[kk]);
-          const int t = kk - n;
-          return ghat_at<IN, E>(a, base + roff[i] + t, i == rc && t == tcen);
-        },
-        [&](int kk, int j) {
-          return kk < n ? R1[(size_t)kk * n + j] : R2[(size_t)(kk - n) * n + j];
-        },
+    tile_gemm<R, true, true>(
+        M, n, n, [&](int i, int kk) { return PE[(size_t)i * n + kk]; },
+        [&](int kk, int j) { return V[(size_t)j * n + kk]; },
         [&](int i, int j, R v) { atomic_add_(gin + base + roff[i] + j, (IN)(v + UV[j] * invM)); },
         As, Bs);
   }
+}
+
+// Back-transformation of every tridiagonal eigenvector at once: Y = Q Z with
+// Q = H_0 ... H_{n-2} from the tridiagonalisation (packed Householder rows +
+// tau), Z the n x n eigenvectors of the tridiagonal.  Thread t owns column
+// t >> 1 and rows 2q + (t & 1) (q < 64) of Z in registers; the two lanes of a
+// column combine their halves of each dot product with one DPP swap, so no
+// barrier is needed per reflector.  Reflectors are staged in LDS, kBtKR at a time.
+constexpr int kBtKR = 32;
+
+struct BtArgs {
+  int np, n;
+  const void* H;    // np x packed Householder rows
+  const void* TAU;  // np x n
+  const void* Z;    // np x n x n (row-major, column c = eigenvector c)
+  void* Y;          // np x n x n
+};
+
+template <class R>
+__global__ __launch_bounds__(kNT) void backtransform_kernel(BtArgs a) {
+  const int n = a.n, tid = threadIdx.x;
+  const int j = tid >> 1, h = tid & 1;
+  __shared__ __align__(16) R vs[2][kBtKR][64];
+  __shared__ R ts[kBtKR];
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    const R* Hp = reinterpret_cast<const R*>(a.H) + (size_t)p * n * n;
+    const R* Tp = reinterpret_cast<const R*>(a.TAU) + (size_t)p * n;
+    const R* Zp = reinterpret_cast<const R*>(a.Z) + (size_t)p * n * n;
+    R* Yp = reinterpret_cast<R*>(a.Y) + (size_t)p * n * n;
+    R y[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int r = 2 * q + h;
+      y[q] = (r < n && j < n) ? Zp[(size_t)r * n + j] : R(0);
+    }
+    for (int kh = n - 2; kh >= 0; kh -= kBtKR) {  // reflectors kh, kh-1, ..., kh-kBtKR+1
+      __syncthreads();
+      for (int e = tid; e < kBtKR * 128; e += kNT) {
+        const int kk = e / 128, r = e % 128, k = kh - kk;
+        R v = R(0);
+        if (k >= 0 && r < n) {
+          const R* row = Hp + hh_row(k, n) - (k + 1);
+          v = r > k + 1 ? row[r] : (r == k + 1 ? R(1) : R(0));
+        }
+        vs[r & 1][kk][r >> 1] = v;
+      }
+      if (tid < kBtKR) ts[tid] = (kh - tid >= 0) ? Tp[kh - tid] : R(0);
+      __syncthreads();
+      const int kcount = min(kBtKR, kh + 1);
+      for (int kk = 0; kk < kcount; ++kk) {
+        const R* v = vs[h][kk];
+        R sdot = R(0);
+#pragma unroll
+        for (int q = 0; q < 64; ++q) sdot = fma(v[q], y[q], sdot);
+        sdot += dpp_f<0xB1>(sdot);
+        const R ts_ = ts[kk] * sdot;
+#pragma unroll
+        for (int q = 0; q < 64; ++q) y[q] = fma(-ts_, v[q], y[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int r = 2 * q + h;
+      if (r < n && j < n) Yp[(size_t)r * n + j] = y[q];
+    }
+  }
+}
+
+template <class R>
+int launch_backtransform(const Plan& pl, const BtArgs& ba, hipStream_t st) {
+  backtransform_kernel<R><<<grid_for(ba.np, 2), kNT, 0, st>>>(ba);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
 }
 
 template <class IN, class E>
@@ -449,7 +641,7 @@ int launch_adj_patch_kernel(const Plan& pl, void (*kern)(const IN*, AdjArgs), co
 template <class E, class RT>
 int launch_adj_spectral(const Plan& pl, const AdjArgs& aa, hipStream_t st) {
   using R = typename real_of<E>::type;
-  const size_t lds = gemm_lds_bytes<R>() + 5 * (((size_t)pl.n * 8 + 15) / 16) * 16 + 64;
+  const size_t lds = gemm_lds_bytes<R>() + 6 * (((size_t)pl.n * 8 + 15) / 16) * 16 + 64;
   auto kern = adj_spectral_kernel<E, RT>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   kern<<<grid_for(aa.np, 2), kNT, lds, st>>>(aa);

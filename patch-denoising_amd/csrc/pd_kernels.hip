@@ -332,6 +332,21 @@ __device__ __forceinline__ T seq_add(T s, const T* a, int count, int stride) {
   return s;
 }
 
+// s + a[off[0]] + a[off[1]] + ... in that order, 8 loads in flight.
+template <class T>
+__device__ __forceinline__ T seq_add(T s, const T* a, const int64_t* off, int count) {
+  int r = 0;
+  for (; r + 8 <= count; r += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = a[off[r + u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; r < count; ++r) s += a[off[r]];
+  return s;
+}
+
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 struct GramArgs {
@@ -689,11 +704,19 @@ template <> struct Lim<double> {
 
 __device__ __forceinline__ float fdiv_(float a, float b) { return __fdividef(a, b); }
 __device__ __forceinline__ double fdiv_(double a, double b) { return a / b; }
+// a / b from the hardware reciprocal refined by one Newton step (a few ulp,
+// no special-case handling; b is kept away from 0 by the callers' pivmin).
+__device__ __forceinline__ float qdiv_(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double qdiv_(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return a * r;
+}
 
 // Number of eigenvalues of the tridiagonal (d, e^2) strictly below x.
 // d and e2 are wave-uniform global arrays (scalar loads); 8 entries are
 // fetched per block so the recurrence never waits on memory.
-template <class R>
+template <class R, bool QD = false>
 __device__ __forceinline__ int sturm_g(const R* __restrict__ d, const R* __restrict__ e2, int n,
                                        R x, R pivmin) {
   R q = d[0] - x;
@@ -709,13 +732,13 @@ __device__ __forceinline__ int sturm_g(const R* __restrict__ d, const R* __restr
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      q = (dv[u] - x) - fdiv_(ev[u], q);
+      q = (dv[u] - x) - (QD ? qdiv_(ev[u], q) : fdiv_(ev[u], q));
       if (fabs(q) < pivmin) q = -pivmin;
       c += q < R(0);
     }
   }
   for (; i < n; ++i) {
-    q = (d[i] - x) - fdiv_(e2[i - 1], q);
+    q = (d[i] - x) - (QD ? qdiv_(e2[i - 1], q) : fdiv_(e2[i - 1], q));
     if (fabs(q) < pivmin) q = -pivmin;
     c += q < R(0);
   }
@@ -750,7 +773,7 @@ struct WaveScratch {
 
 // Eigenvalues of the tridiagonal at ascending indices tgt[0..nt) by
 // multisection: the 64 lanes are split into groups of L = 64/nt points.
-template <class R>
+template <class R, bool QD = false>
 __device__ void multisect(WaveScratch<R>& s, const R* __restrict__ d, const R* __restrict__ e2,
                           int n, R pivmin, R glo, R ghi, R atol, int nt) {
   const int lane = threadIdx.x;
@@ -771,7 +794,7 @@ __device__ void multisect(WaveScratch<R>& s, const R* __restrict__ d, const R* _
       const bool done = !active || (hi - lo) <= tol || mid <= lo || mid >= hi;
       if (__all(done)) break;
       const R x = lo + (hi - lo) * (R)(sub + 1) / (R)(L + 1);
-      const int c = sturm_g(d, e2, n, x, pivmin);
+      const int c = sturm_g<R, QD>(d, e2, n, x, pivmin);
       s.xs[lane] = x;
       s.cs[lane] = c;
       wave_sync();
@@ -796,9 +819,10 @@ __device__ void multisect(WaveScratch<R>& s, const R* __restrict__ d, const R* _
 // Solve (T - lam I) x = b for the tridiagonal (d, e) with partial pivoting.
 // d / e are wave-uniform global arrays (e[n-1] = 0); b is in rhs (stride ks)
 // and the solution overwrites it; u0 / u1 / u2 (stride ks) hold U.
-template <class R>
+template <class R, bool QD = false>
 __device__ void tri_solve(const R* __restrict__ d, const R* __restrict__ e, int n, R lam, R* u0,
                           R* u1, R* u2, R* rhs, int ks, R eps_piv) {
+  auto dv_ = [](R a, R b) { return QD ? qdiv_(a, b) : fdiv_(a, b); };
   R c0 = d[0] - lam, c1 = e[0], c2 = R(0), r = rhs[0];
   auto step = [&](int i, R n0, R n1, R n2, R nr) {
     if (fabs(n0) > fabs(c0)) {
@@ -808,8 +832,8 @@ __device__ void tri_solve(const R* __restrict__ d, const R* __restrict__ e, int 
       t = c2; c2 = n2; n2 = t;
       t = r; r = nr; nr = t;
     }
-    if (c0 == R(0)) c0 = eps_piv;
-    const R f = fdiv_(n0, c0);
+    if (QD ? fabs(c0) < eps_piv : c0 == R(0)) c0 = QD ? copysign(eps_piv, c0) : eps_piv;
+    const R f = dv_(n0, c0);
     u0[i * ks] = c0;
     u1[i * ks] = c1;
     u2[i * ks] = c2;
@@ -833,10 +857,10 @@ __device__ void tri_solve(const R* __restrict__ d, const R* __restrict__ e, int 
     for (int u = 0; u < 4; ++u) step(i + u, ev[u], dv[u] - lam, fv[u], bv[u]);
   }
   for (; i < n - 1; ++i) step(i, e[i], d[i + 1] - lam, e[i + 1], rhs[(i + 1) * ks]);
-  if (c0 == R(0)) c0 = eps_piv;
+  if (QD ? fabs(c0) < eps_piv : c0 == R(0)) c0 = QD ? copysign(eps_piv, c0) : eps_piv;
   u0[(n - 1) * ks] = c0;
   // back substitution
-  R z2 = R(0), z1 = fdiv_(r, c0);
+  R z2 = R(0), z1 = dv_(r, c0);
   rhs[(n - 1) * ks] = z1;
   int k = n - 2;
   for (; k - 3 >= 0; k -= 4) {
@@ -850,14 +874,14 @@ __device__ void tri_solve(const R* __restrict__ d, const R* __restrict__ e, int 
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const R z = fdiv_(b[u] - a1[u] * z1 - a2[u] * z2, a0[u]);
+      const R z = dv_(b[u] - a1[u] * z1 - a2[u] * z2, a0[u]);
       rhs[(k - u) * ks] = z;
       z2 = z1;
       z1 = z;
     }
   }
   for (; k >= 0; --k) {
-    const R z = fdiv_(rhs[k * ks] - u1[k * ks] * z1 - u2[k * ks] * z2, u0[k * ks]);
+    const R z = dv_(rhs[k * ks] - u1[k * ks] * z1 - u2[k * ks] * z2, u0[k * ks]);
     rhs[k * ks] = z;
     z2 = z1;
     z1 = z;
@@ -884,8 +908,115 @@ __device__ double shrink(int loss, double y, double beta) {
   return sqrt(fmax(t * t - 4.0 * beta, 0.0) / y);
 }
 
+// Every eigenvector of the tridiagonal (d, e), eigenvalues lam[0..n) in
+// ascending order, into Z (row stride ks, column c = vector c).  Lane c of the
+// wavefront handles component c (64 at a time) with a twisted factorization
+// (the eigenvector step of the MRRR algorithm, Dhillon & Parlett 2004): the
+// top-down L+ D+ L+^T and bottom-up U- D- U-^T factorizations of T - lam I
+// (pivots kept at least eps_piv in magnitude), the twist index r minimising
+// |gamma_r| = |D+_r + D-_r - (d_r - lam)|, then z_r = 1 and the two
+// recurrences z_i = -L+_i z_{i+1} (i < r), z_i = -U-_i z_{i-1} (i > r).  This
+// is one step of inverse iteration from e_r, with factors in scr (3 n x n);
+// a non-finite result falls back to two inverse-iteration solves.  The
+// wavefront then re-orthogonalises (modified Gram-Schmidt, twice) the vectors
+// of eigenvalue clusters closer than clus.  For the back-transformation by a
+// separate kernel.
+template <class R>
+__device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__ e, int n,
+                                  const R* lam, R* Z, R* scr, int ks, R eps_piv, R clus) {
+  const int lane = threadIdx.x;
+  for (int c0 = 0; c0 < n; c0 += kWave) {
+    const int c = c0 + lane;
+    if (c < n) {
+      R* z = Z + c;
+      R* dp = scr + c;                   // D+ of L+ D+ L+^T = T - lam I (top-down)
+      R* lp = scr + (size_t)n * ks + c;  // L+
+      R* um = scr + (size_t)2 * n * ks + c;  // U- of U- D- U-^T (bottom-up)
+      const R lm = lam[c];
+      auto guard = [&](R x) { return fabs(x) < eps_piv ? copysign(eps_piv, x) : x; };
+      // twisted factorization: z = (T - lam I)^-1 e_r up to scale, r = argmin |gamma_r|
+      R dv = guard(d[0] - lm);
+      dp[0] = dv;
+      for (int i = 0; i + 1 < n; ++i) {
+        const R l = qdiv_(e[i], dv);
+        lp[(size_t)i * ks] = l;
+        dv = guard((d[i + 1] - lm) - l * e[i]);
+        dp[(size_t)(i + 1) * ks] = dv;
+      }
+      R dm = guard(d[n - 1] - lm);
+      R best = fabs(dp[(size_t)(n - 1) * ks] + dm - (d[n - 1] - lm));
+      int r = n - 1;
+      for (int i = n - 2; i >= 0; --i) {
+        const R u = qdiv_(e[i], dm);
+        um[(size_t)(i + 1) * ks] = u;
+        dm = guard((d[i] - lm) - u * e[i]);
+        const R g = fabs(dp[(size_t)i * ks] + dm - (d[i] - lm));
+        if (g < best) {
+          best = g;
+          r = i;
+        }
+      }
+      z[(size_t)r * ks] = R(1);
+      R zi = R(1), ss = R(1);
+      for (int i = r - 1; i >= 0; --i) {
+        zi = -lp[(size_t)i * ks] * zi;
+        z[(size_t)i * ks] = zi;
+        ss += zi * zi;
+      }
+      zi = R(1);
+      for (int i = r + 1; i < n; ++i) {
+        zi = -um[(size_t)i * ks] * zi;
+        z[(size_t)i * ks] = zi;
+        ss += zi * zi;
+      }
+      if (!(ss < R(INFINITY))) {
+        // fallback: inverse iteration from a fixed pseudo-random start
+        uint32_t hs = 0x9E3779B9u * (uint32_t)(c + 1);
+        for (int i = 0; i < n; ++i) {
+          hs ^= hs << 13; hs ^= hs >> 17; hs ^= hs << 5;
+          z[(size_t)i * ks] = R(0.5) + (R)(hs & 0xFFFF) * R(1.0 / 65536.0);
+        }
+        for (int it = 0; it < 2; ++it) {
+          tri_solve<R, true>(d, e, n, lm, dp, lp, um, z, ks, eps_piv);
+          R mx = R(0);
+          for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(z[(size_t)i * ks]));
+          const R inv = mx > R(0) ? R(1) / mx : R(1);
+          for (int i = 0; i < n; ++i) z[(size_t)i * ks] *= inv;
+        }
+        ss = R(0);
+        for (int i = 0; i < n; ++i) ss += z[(size_t)i * ks] * z[(size_t)i * ks];
+      }
+      const R inv = R(1) / sqrt(fmax(ss, Lim<R>::tiny()));
+      for (int i = 0; i < n; ++i) z[(size_t)i * ks] *= inv;
+    }
+  }
+  __syncthreads();  // columns written lane-wise, read row-wise below
+  const int i0 = lane, i1 = lane + kWave;
+  for (int c = 1; c < n; ++c) {
+    int qs = c;
+    while (qs > 0 && fabs(lam[qs - 1] - lam[c]) <= clus) --qs;
+    if (qs == c) continue;
+    R zc0 = i0 < n ? Z[(size_t)i0 * ks + c] : R(0);
+    R zc1 = i1 < n ? Z[(size_t)i1 * ks + c] : R(0);
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int q = qs; q < c; ++q) {
+        const R zq0 = i0 < n ? Z[(size_t)i0 * ks + q] : R(0);
+        const R zq1 = i1 < n ? Z[(size_t)i1 * ks + q] : R(0);
+        const R dot = dpp_sum(zq0 * zc0 + zq1 * zc1);
+        zc0 -= dot * zq0;
+        zc1 -= dot * zq1;
+      }
+      const R inv = R(1) / sqrt(fmax(dpp_sum(zc0 * zc0 + zc1 * zc1), Lim<R>::tiny()));
+      zc0 *= inv;
+      zc1 *= inv;
+    }
+    if (i0 < n) Z[(size_t)i0 * ks + c] = zc0;
+    if (i1 < n) Z[(size_t)i1 * ks + c] = zc1;
+  }
+  __syncthreads();
+}
+
 constexpr int kKB = 16;  // components per eigenvector batch (registers per lane)
-constexpr int kAllPairs = 16;  // internal eigen-stage rule: every eigenpair, ascending
 
 // One wavefront per patch.  Eigenvalues by multisection on the tridiagonal,
 // the method's rule, then for the kept components: inverse iteration (one
@@ -1041,20 +1172,6 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       wave_sync();
       ncomp = sh_i[0];
       rank = sh_i[1];
-    } else if (a.method == kAllPairs) {
-      // every eigenpair, ascending (recomputation for the backward pass);
-      // WC holds the eigenvalues
-      for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
-      wave_sync();
-      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
-      for (int c = lane; c < n; c += kWave) {
-        cidx[c] = c;
-        lam[c] = s.val[c];
-        wc[c] = (double)s.val[c];
-      }
-      ncomp = n;
-      rank = 0;
-      wave_sync();
     } else {
       // ---- optimal shrinkage, lowrank.py:387-407
       double sigma;
@@ -1140,7 +1257,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
     // orthogonal to ~ulp * tnorm / gap, so FP64 needs it only for gaps below
     // 1e-8 tnorm (error <= ~1e-8); FP32 keeps the classical 1e-3 tnorm.
     const R clus = (sizeof(R) == 8 ? R(1e-8) : R(1e-3)) * tnorm;
-    const int n_init = a.method == kAllPairs ? 2 : 3;  // initial solves per vector
+    const int n_init = 3;  // initial solves per vector
     const int i0 = lane, i1 = lane + kWave;  // rows held by this lane
     for (int c0 = 0; c0 < ncomp; c0 += KB) {
       const int kb = min(KB, ncomp - c0);
@@ -1176,8 +1293,13 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         for (int c = 0; c < kKB; ++c) {
           if (c >= kb) continue;
           const int gc = c0 + c;
-          // earlier batches (stored z in global memory)
-          for (int q = 0; q < c0; ++q) {
+          // earlier batches (stored z in global memory).  The components are
+          // ordered monotonically by eigenvalue within each run (descending
+          // top, ascending bottom / all), so the cluster of gc among them is
+          // the contiguous run below c0 found by this scan.
+          int qs = c0;
+          while (qs > 0 && fabs(lam[qs - 1] - lam[gc]) <= clus) --qs;
+          for (int q = qs; q < c0; ++q) {
             if (fabs(lam[q] - lam[gc]) > clus) continue;
             const R zq0 = i0 < n ? Zp[(size_t)i0 * a.kmax + q] : R(0);
             const R zq1 = i1 < n ? Zp[(size_t)i1 * a.kmax + q] : R(0);
@@ -1297,6 +1419,63 @@ size_t trieig_lds_bytes(int n, int kb) {
   s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(double));
   s += al((size_t)n * kb * sizeof(R));
   return s;
+}
+
+// All eigenpairs of each patch's tridiagonal (recomputation for the backward
+// pass): multisection for every eigenvalue (ascending, into WC), then
+// all_pairs_vectors into Z.  A lean one-wavefront kernel: no threshold rules,
+// no register-held batches, so many wavefronts share a SIMD and hide the
+// latency of the Sturm and LU recurrences.
+template <class R>
+__global__ __launch_bounds__(kWave) void trieig_all_kernel(EigArgs a) {
+  const int lane = threadIdx.x;
+  const int n = a.n;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  WaveScratch<R> s;
+  s.lo = lds_carve<R>(cur, kWave);
+  s.hi = lds_carve<R>(cur, kWave);
+  s.xs = lds_carve<R>(cur, kWave);
+  s.cs = lds_carve<int>(cur, kWave);
+  s.tgt = lds_carve<int>(cur, n + 16);
+  s.val = lds_carve<R>(cur, n + 16);
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    const R* __restrict__ dg = reinterpret_cast<const R*>(a.DE) + (size_t)p * 3 * n;
+    const R* __restrict__ eg = dg + n;
+    const R* __restrict__ e2g = dg + 2 * n;
+    R glo = R(INFINITY), ghi = R(-INFINITY), emax2 = R(0);
+    for (int i = lane; i < n; i += kWave) {
+      const R off = (i > 0 ? fabs(eg[i - 1]) : R(0)) + fabs(eg[i]);
+      glo = fmin(glo, dg[i] - off);
+      ghi = fmax(ghi, dg[i] + off);
+      emax2 = fmax(emax2, e2g[i]);
+    }
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+      glo = fmin(glo, shfl_xor_(glo, m));
+      ghi = fmax(ghi, shfl_xor_(ghi, m));
+      emax2 = fmax(emax2, shfl_xor_(emax2, m));
+    }
+    const R tnorm = fmax(fabs(glo), fabs(ghi));
+    const R pivmin = Lim<R>::tiny() * fmax(R(1), emax2);
+    const R atol = Lim<R>::ulp() * tnorm * R(0.5);
+    glo -= R(4) * Lim<R>::ulp() * tnorm + R(4) * pivmin;
+    ghi += R(4) * Lim<R>::ulp() * tnorm + R(4) * pivmin;
+    for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
+    wave_sync();
+    multisect<R, true>(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
+    for (int c = lane; c < n; c += kWave) a.WC[(size_t)p * a.kmax + c] = (double)s.val[c];
+    const R eps_piv = Lim<R>::ulp() * tnorm + pivmin;
+    const R clus = (sizeof(R) == 8 ? R(1e-8) : R(1e-3)) * tnorm;
+    all_pairs_vectors(dg, eg, n, (const R*)s.val, reinterpret_cast<R*>(a.Z) + (size_t)p * n * a.kmax,
+                      reinterpret_cast<R*>(a.SCR) + (size_t)p * 3 * n * n, a.kmax, eps_piv, clus);
+  }
+}
+
+template <class R>
+size_t trieig_all_lds_bytes(int n) {
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  return 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int)) + al((n + 16) * sizeof(int)) +
+         al((n + 16) * sizeof(R));
 }
 
 // ===========================================================================
@@ -1682,8 +1861,8 @@ struct Plan {
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
       off_WC, off_NC, off_TH, off_w3, off_SCR, total;
   // backward only
-  size_t off_HC = 0, off_GS = 0, off_S1 = 0, off_S2 = 0, off_R1 = 0, off_R2 = 0, off_UV = 0,
-         off_THB = 0;
+  size_t off_P1 = 0, off_P2 = 0, off_PE = 0, off_GS = 0, off_S1 = 0, off_UV = 0, off_THB = 0,
+         off_GH = 0;
 };
 
 // bwd: layout for pdamd_optimal_backward (all eigenpairs, adjoint buffers).
@@ -1756,7 +1935,7 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   // chunk: keep the per-chunk buffers under ~1 GiB (4 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
-  if (bwd) per += 5 * (size_t)n * n * pl->ebytes + 2 * (size_t)n * pl->ebytes + 8;
+  if (bwd) per += (3 * (size_t)M * n + (size_t)n * n + 2 * (size_t)n) * pl->ebytes + 8;
   int64_t chunk = (int64_t)((bwd ? (4ull << 30) : (1ull << 30)) / per);
   chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
   pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
@@ -1776,7 +1955,7 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   const size_t C = (size_t)pl->chunk;
   pl->off_H = take(C * n * n * pl->ebytes);
   pl->off_DE = take(C * 3 * n * pl->rbytes);
-  pl->off_SCR = take(C * 3 * n * pl->kb * pl->rbytes);
+  pl->off_SCR = take(C * 3 * n * (bwd ? n : pl->kb) * pl->rbytes);
   pl->off_TAU = take(C * n * pl->ebytes);
   pl->off_Y = take(C * n * pl->kmax * pl->ebytes);
   pl->off_Z = take(C * n * pl->kmax * pl->rbytes);
@@ -1785,12 +1964,12 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   pl->off_TH = take(C * sizeof(double));
   pl->off_w3 = take((size_t)pl->nvox * (g.gt == 1 ? 1 : g.T) * sizeof(double));
   if (bwd) {
-    pl->off_HC = take(C * n * n * pl->ebytes);
+    pl->off_GH = take((size_t)pl->nvox * g.T * pl->ebytes);
+    pl->off_P1 = take(C * M * n * pl->ebytes);
+    pl->off_P2 = take(C * M * n * pl->ebytes);
+    pl->off_PE = take(C * M * n * pl->ebytes);
     pl->off_GS = take(C * n * pl->ebytes);
     pl->off_S1 = take(C * n * n * pl->ebytes);
-    pl->off_S2 = take(C * n * n * pl->ebytes);
-    pl->off_R1 = take(C * n * n * pl->ebytes);
-    pl->off_R2 = take(C * n * n * pl->ebytes);
     pl->off_UV = take(C * n * pl->ebytes);
     pl->off_THB = take(C * sizeof(double));
   }
@@ -1888,6 +2067,15 @@ int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(16, (int)((160 * 1024) / lds)));
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class R>
+int launch_eig_all(const Plan& pl, EigArgs ea, hipStream_t st) {
+  const size_t lds = trieig_all_lds_bytes<R>(pl.n);
+  auto kern = trieig_all_kernel<R>;
+  kern<<<grid_for(ea.np, 32), kWave, lds, st>>>(ea);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }
@@ -2014,6 +2202,21 @@ template <class IN, class E, class RT>
 int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, const void* gout,
                  const float* w4, void* gin, double* glam, hipStream_t st, StageTimer& tm) {
   const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
+  {
+    AdjArgs ha;
+    ha.g = pl.g;
+    ha.recomb = P->recombination;
+    ha.mask_kind = P->mask_kind;
+    ha.mask = P->mask_dev;
+    ha.gout = gout;
+    ha.w4 = w4;
+    ha.GH = ws + pl.off_GH;
+    const int64_t total = pl.nvox * pl.g.T;
+    tm.begin(PDAMD_STAGE_ADJ_CROSS);
+    ghat_kernel<IN, E><<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(ha, total);
+    tm.end();
+    HIP_TRY(hipGetLastError());
+  }
   for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
     const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
     // recompute the Gram eigendecomposition (all pairs)
@@ -2027,11 +2230,20 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
     tm.end();
     if (rc) return rc;
     EigArgs ea = eig_args(P, pl, ws, ga);
-    ea.method = kAllPairs;
     ea.var_map = nullptr;
     ea.LAM = nullptr;
     tm.begin(PDAMD_STAGE_EIG);
-    rc = launch_eig<E, RT>(pl, ea, st);
+    rc = launch_eig_all<E>(pl, ea, st);
+    if (!rc) {
+      BtArgs ba;
+      ba.np = np;
+      ba.n = pl.n;
+      ba.H = ga.H;
+      ba.TAU = ga.TAU;
+      ba.Z = ea.Z;
+      ba.Y = ea.Y;
+      rc = launch_backtransform<E>(pl, ba, st);
+    }
     tm.end();
     if (rc) return rc;
     AdjArgs aa;
@@ -2044,14 +2256,14 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
     aa.mask = P->mask_dev;
     aa.gout = gout;
     aa.w4 = w4;
-    aa.HC = ws + pl.off_HC;
-    aa.GS = ws + pl.off_GS;
+    aa.GH = ws + pl.off_GH;
     aa.Y = ea.Y;
     aa.EV = ea.WC;
+    aa.P1 = ws + pl.off_P1;
+    aa.P2 = ws + pl.off_P2;
+    aa.PE = ws + pl.off_PE;
+    aa.GS = ws + pl.off_GS;
     aa.S1 = ws + pl.off_S1;
-    aa.S2 = ws + pl.off_S2;
-    aa.R1 = ws + pl.off_R1;
-    aa.R2 = ws + pl.off_R2;
     aa.UV = ws + pl.off_UV;
     aa.THB = reinterpret_cast<double*>(ws + pl.off_THB);
     aa.loss = P->loss;
@@ -2062,7 +2274,7 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
     aa.GLAM = glam;
     aa.gin = gin;
     tm.begin(PDAMD_STAGE_ADJ_CROSS);
-    rc = launch_adj_patch_kernel<IN, E>(pl, adj_cross_kernel<IN, E>, P->input_dev, aa, st);
+    rc = launch_adj_patch_kernel<IN, E>(pl, adj_project_kernel<IN, E>, P->input_dev, aa, st);
     tm.end();
     if (rc) return rc;
     tm.begin(PDAMD_STAGE_ADJ_SPECTRAL);
