@@ -1,19 +1,23 @@
 // Space-time low-rank patch denoising on MI355X (gfx950).
 //
-// Pipeline for one volume (all launches on the caller's stream):
+// Forward pipeline for one volume (all launches on the caller's stream):
 //   1. mask_window_kernel / select_kernel / scan / compact   patch selection
-//   2. gram_tridiag_kernel   per patch: gather the Casorati matrix, centre it
-//                            (per-frame spatial mean, same summation order as
-//                            NumPy), form the min-side Gram in registers, then
-//                            Householder-tridiagonalise it in LDS
-//   3. trieig_kernel         per patch, one wavefront: multisection on the
-//                            tridiagonal (Sturm counts), the method's threshold
-//                            rule, inverse iteration for the kept components,
-//                            back-transformation to Gram eigenvectors
-//   4. synth_kernel          per patch: re-gather, project on the kept
-//                            components, scatter-accumulate theta * patch into
-//                            the output and theta into the weights
-//   5. finalize_kernel       out /= weights, masked voxels -> 0
+//   2. gram_kernel      per patch: gather the Casorati matrix, centre it
+//                       (per-frame spatial mean, same summation order as
+//                       NumPy), form the min-side Gram (FP32 on MFMA 32x32x2)
+//   3. tridiag_kernel   per patch: register-resident Householder
+//                       tridiagonalisation of the Gram
+//   4. trieig_kernel    per patch, one wavefront: multisection on the
+//                       tridiagonal (Sturm counts), the method's threshold
+//                       rule, inverse iteration for the kept components,
+//                       back-transformation to Gram eigenvectors
+//   5. synth_kernel     per patch: re-gather, project on the kept
+//                       components, scatter-accumulate theta * patch into
+//                       the output and theta into float64 weight sums
+//   6. finalize_kernel  float32 weights, out /= weights, masked voxels -> 0
+// Backward of the optimal rule (pd_adjoint.inl): steps 1-3 again, then
+// trieig_all_kernel + backtransform_kernel (every eigenpair), ghat_kernel,
+// adj_project_kernel, adj_spectral_kernel, adj_synth_kernel.
 #include <float.h>
 #include <math.h>
 #include <stdarg.h>
