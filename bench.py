@@ -133,6 +133,47 @@ def bench_differentiable(steps):
                                    if cnt[i]})
 
 
+def bench_batched(steps):
+    """The eight-volume mixed workload (seed 4; volumes whose Gram size
+    exceeds 128 are listed as skipped): one pdamd_denoise call per volume vs
+    one pdamd_denoise_batch call (eager, and replayed from its captured
+    graph), inputs resident in HBM."""
+    import torch
+
+    from patch_denoise import _native as N
+    from patch_denoise import workloads
+    from patch_denoise.batch import BatchPlan
+
+    t0 = time.perf_counter()
+    vols = workloads.config4()
+    calls, skipped = workloads.config4_calls(vols)
+    gen_s = time.perf_counter() - t0
+    eager = BatchPlan(calls, use_graph=False)
+    graph = BatchPlan(calls, use_graph=True)
+    singles = [N.DeviceRun(j.prob, j.data, j.mask_arr, j.var_map) for j in eager.jobs]
+    vf = float(sum(np.prod(j.data.shape) for j in eager.jobs))
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    t_seq = timed(lambda: [r.run() for r in singles])
+    t_eager = timed(eager.run)
+    t_graph = timed(graph.run)
+    return dict(workload="batched_8vol_seed4", volumes=[
+        dict(method=c[0], shape=list(c[1][0].shape), patch=c[1][1]) for c in calls],
+        skipped=[dict(index=i, reason=r) for i, r in skipped], voxel_frames=vf,
+        patches=eager.batch.selected, steps=steps, datagen_s=gen_s,
+        sequential_ms=t_seq * 1e3, batch_ms=t_eager * 1e3, batch_graph_ms=t_graph * 1e3,
+        sequential_voxel_frames_s=vf / t_seq, batch_graph_voxel_frames_s=vf / t_graph,
+        speedup_graph_vs_sequential=t_seq / t_graph)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +182,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="auto")
+    ap.add_argument("--batch-steps", type=int, default=3,
+                    help="steps of the batched multi-volume measurement; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")
     ap.add_argument("--calib-copy", action="store_true",
@@ -247,6 +290,7 @@ def main():
                    host_cpus=os.cpu_count())
 
     diff = bench_differentiable(args.diff_steps) if args.diff_steps > 0 else None
+    batched = bench_batched(args.batch_steps) if args.batch_steps > 0 else None
 
     line = {
         "metric": "MP-PCA voxel-frames/s at 80x80x48x300",
@@ -269,6 +313,7 @@ def main():
         "stages": stages,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "differentiable": diff,
+        "batched": batched,
         "datagen_s": gen_s,
     }
     print(json.dumps(line))

@@ -153,6 +153,22 @@ int pdamd_denoise(pdamd_problem* prob, void* workspace_dev, size_t workspace_byt
 int pdamd_select_patches(const pdamd_problem* prob, int32_t* index_dev, int64_t* count,
                          void* workspace_dev, size_t workspace_bytes, void* stream);
 
+/* Batches of independent volumes (replaces a Python loop of
+   BaseSpaceTimeDenoiser.denoise calls, space_time/base.py:177-270, one per
+   volume).  Every volume is zeroed and selected on the caller's stream, all
+   patch counts are read with ONE host synchronisation, then each volume's
+   Gram / eigen / synthesis chain runs on its own stream forked from the
+   caller's stream (up to 8 concurrently) and joined back to it.  With
+   use_graph != 0 that post-selection work is captured into a hipGraph the
+   first time a given set of problems (fields, pointers and patch counts) is
+   seen and replayed afterwards (a small per-process cache).  Each problem's
+   `selected` is set; stage timing is not collected in batches.  The
+   workspace is the sum of the per-volume workspaces (each rounded up to
+   256 bytes).                                                              */
+int pdamd_batch_workspace_size(const pdamd_problem* probs, int count, size_t* bytes);
+int pdamd_denoise_batch(pdamd_problem* probs, int count, void* workspace_dev,
+                        size_t workspace_bytes, int use_graph, void* stream);
+
 /* Backward of OPTIMAL shrinkage (weighted / average / center recombination,
    real data): given dL/d(output) (grad_out_dev, input dtype) and the forward's
    weights (weights_dev), accumulate dL/d(input) into grad_in_dev (input dtype,

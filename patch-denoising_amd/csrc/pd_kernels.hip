@@ -1981,8 +1981,9 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   return PDAMD_OK;
 }
 
-int run_select(const pdamd_problem* P, const Plan& pl, char* ws, int32_t* idx_out,
-               int64_t* count_host, hipStream_t st, StageTimer* tm = nullptr) {
+// Enqueue the selection kernels; the patch count lands in ws + off_count.
+int run_select_async(const pdamd_problem* P, const Plan& pl, char* ws, int32_t* idx_out,
+                     hipStream_t st, StageTimer* tm = nullptr) {
   int* cnt = reinterpret_cast<int*>(ws + pl.off_cnt);
   uint8_t* flags = reinterpret_cast<uint8_t*>(ws + pl.off_flags);
   int* btot = reinterpret_cast<int*>(ws + pl.off_btot);
@@ -2004,8 +2005,33 @@ int run_select(const pdamd_problem* P, const Plan& pl, char* ws, int32_t* idx_ou
   compact_kernel<<<nblk, kNT, 0, st>>>(flags, pl.npatch, btot, idx_out);
   if (tm) tm->end();
   HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+int run_select(const pdamd_problem* P, const Plan& pl, char* ws, int32_t* idx_out,
+               int64_t* count_host, hipStream_t st, StageTimer* tm = nullptr) {
+  const int rc = run_select_async(P, pl, ws, idx_out, st, tm);
+  if (rc) return rc;
+  const int64_t* dcount = reinterpret_cast<const int64_t*>(ws + pl.off_count);
   HIP_TRY(hipMemcpyAsync(count_host, dcount, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  return PDAMD_OK;
+}
+
+// Zero fill by a kernel (capturable; used where hipMemsetAsync could end up in
+// a graph).  bytes must be a multiple of 4.
+__global__ void zero_kernel(uint32_t* __restrict__ p, int64_t nwords) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int zero_async(void* p, size_t bytes, hipStream_t st) {
+  const int64_t nw = (int64_t)(bytes / 4);
+  if (nw == 0) return PDAMD_OK;
+  zero_kernel<<<(int)std::min<int64_t>((nw + 255) / 256, 65536), 256, 0, st>>>(
+      reinterpret_cast<uint32_t*>(p), nw);
+  HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }
 
@@ -2293,6 +2319,75 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
   return PDAMD_OK;
 }
 
+int dispatch_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
+                    StageTimer& tm) {
+  switch (P->dtype * 4 + pl.etype) {
+    case PDAMD_F32 * 4 + 0: return run_chunks<float, float, float>(P, pl, ws, count, st, tm);
+    case PDAMD_F32 * 4 + 1: return run_chunks<float, double, float>(P, pl, ws, count, st, tm);
+    case PDAMD_F64 * 4 + 1: return run_chunks<double, double, double>(P, pl, ws, count, st, tm);
+    case PDAMD_C64 * 4 + 2:
+      return run_chunks<cpx<float>, cpx<float>, float>(P, pl, ws, count, st, tm);
+    case PDAMD_C64 * 4 + 3:
+      return run_chunks<cpx<float>, cpx<double>, float>(P, pl, ws, count, st, tm);
+    case PDAMD_C128 * 4 + 3:
+      return run_chunks<cpx<double>, cpx<double>, double>(P, pl, ws, count, st, tm);
+    default:
+      return fail(PDAMD_EUNSUPPORTED, "dtype/precision combination not built");
+  }
+}
+
+size_t elem_bytes(int dtype) {
+  return dtype == PDAMD_F32 ? 4 : (dtype == PDAMD_C128 ? 16 : 8);
+}
+
+// ---------------------------------------------------------------------------
+// Batches of volumes: one stream per volume (forked from and joined back to
+// the caller's stream), optional capture of the post-selection work into a
+// cached hipGraph.
+// ---------------------------------------------------------------------------
+constexpr int kBatchStreams = 8;
+
+struct BatchStreams {
+  int dev = -1;
+  hipStream_t s[kBatchStreams] = {};
+  hipEvent_t ev[kBatchStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipStream_t cap = nullptr;  // graph capture / replay stream (the caller's may be the null stream)
+  hipEvent_t pre = nullptr, done = nullptr;
+};
+
+int batch_streams(BatchStreams** out) {
+  static BatchStreams bs[16];
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 16) return fail(PDAMD_EUNSUPPORTED, "device index %d", dev);
+  BatchStreams& b = bs[dev];
+  if (b.dev < 0) {
+    for (int i = 0; i < kBatchStreams; ++i) {
+      HIP_TRY(hipStreamCreateWithFlags(&b.s[i], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&b.ev[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
+    HIP_TRY(hipStreamCreateWithFlags(&b.cap, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b.pre, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    b.dev = dev;
+  }
+  *out = &b;
+  return PDAMD_OK;
+}
+
+struct GraphEntry {
+  int dev;
+  std::vector<char> key;
+  hipGraphExec_t exec;
+};
+
+std::vector<GraphEntry>& graph_cache() {
+  static std::vector<GraphEntry> g;
+  return g;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2350,25 +2445,143 @@ int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes,
   if (P->patch_index_dev && count > 0)
     HIP_TRY(hipMemcpyAsync(P->patch_index_dev, idx, count * sizeof(int32_t),
                            hipMemcpyDeviceToDevice, st));
-  switch (P->dtype * 4 + pl.etype) {
-    case PDAMD_F32 * 4 + 0: rc = run_chunks<float, float, float>(P, pl, ws, count, st, tm); break;
-    case PDAMD_F32 * 4 + 1: rc = run_chunks<float, double, float>(P, pl, ws, count, st, tm); break;
-    case PDAMD_F64 * 4 + 1: rc = run_chunks<double, double, double>(P, pl, ws, count, st, tm); break;
-    case PDAMD_C64 * 4 + 2:
-      rc = run_chunks<cpx<float>, cpx<float>, float>(P, pl, ws, count, st, tm);
-      break;
-    case PDAMD_C64 * 4 + 3:
-      rc = run_chunks<cpx<float>, cpx<double>, float>(P, pl, ws, count, st, tm);
-      break;
-    case PDAMD_C128 * 4 + 3:
-      rc = run_chunks<cpx<double>, cpx<double>, double>(P, pl, ws, count, st, tm);
-      break;
-    default:
-      return fail(PDAMD_EUNSUPPORTED, "dtype/precision combination not built");
-  }
+  rc = dispatch_chunks(P, pl, ws, count, st, tm);
   if (rc) return rc;
   if (tm.collect(P->stage_ms, P->stage_launches))
     return fail(PDAMD_EHIP, "stream failed while collecting stage timings");
+  return PDAMD_OK;
+}
+
+int pdamd_batch_workspace_size(const pdamd_problem* probs, int count, size_t* bytes) {
+  if (!probs || count < 1) return fail(PDAMD_EINVAL, "empty batch");
+  size_t tot = 0;
+  for (int i = 0; i < count; ++i) {
+    Plan pl;
+    const int rc = make_plan(&probs[i], &pl);
+    if (rc) return fail(rc, "volume %d: %s", i, g_err.c_str());
+    tot += ((pl.total + 255) / 256) * 256;
+  }
+  *bytes = tot;
+  return PDAMD_OK;
+}
+
+int pdamd_denoise_batch(pdamd_problem* probs, int count, void* workspace_dev,
+                        size_t workspace_bytes, int use_graph, void* stream) {
+  if (!probs || count < 1) return fail(PDAMD_EINVAL, "empty batch");
+  std::vector<Plan> pls(count);
+  std::vector<size_t> off(count);
+  size_t tot = 0;
+  for (int i = 0; i < count; ++i) {
+    pdamd_problem* P = &probs[i];
+    int rc = make_plan(P, &pls[i]);
+    if (rc) return fail(rc, "volume %d: %s", i, g_err.c_str());
+    if (!P->input_dev || !P->output_dev || !P->weights_dev)
+      return fail(PDAMD_EINVAL, "volume %d: null data pointer", i);
+    if (P->method == PDAMD_HYBRID && !P->var_map_dev)
+      return fail(PDAMD_EINVAL, "volume %d: hybrid rule needs a variance map", i);
+    if (P->lam_dev && P->method != PDAMD_OPTIMAL)
+      return fail(PDAMD_EINVAL, "volume %d: a threshold scale map is only used by the optimal rule", i);
+    off[i] = tot;
+    tot += ((pls[i].total + 255) / 256) * 256;
+  }
+  if (workspace_bytes < tot)
+    return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, tot);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* ws = reinterpret_cast<char*>(workspace_dev);
+  // 1. zero outputs and select, every volume on the caller's stream
+  for (int i = 0; i < count; ++i) {
+    pdamd_problem* P = &probs[i];
+    const Plan& pl = pls[i];
+    char* w = ws + off[i];
+    int rc = zero_async(P->output_dev, (size_t)pl.nvox * pl.g.T * elem_bytes(P->dtype), st);
+    if (!rc && P->recombination == PDAMD_CENTER)
+      rc = zero_async(P->weights_dev, (size_t)pl.nvox * pl.g.T * sizeof(float), st);
+    if (!rc)
+      rc = zero_async(w + pl.off_w3, (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st);
+    if (!rc) rc = run_select_async(P, pl, w, reinterpret_cast<int32_t*>(w + pl.off_idx), st);
+    if (rc) return rc;
+  }
+  // 2. one synchronisation for every patch count
+  std::vector<int64_t> counts(count);
+  for (int i = 0; i < count; ++i)
+    HIP_TRY(hipMemcpyAsync(&counts[i], ws + off[i] + pls[i].off_count, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int i = 0; i < count; ++i) probs[i].selected = counts[i];
+  // 3. per-volume kernel chains on forked streams
+  BatchStreams* bs = nullptr;
+  int rc = batch_streams(&bs);
+  if (rc) return rc;
+  StageTimer off_tm;  // no per-stage timing in batches
+  auto enqueue = [&](hipStream_t st) -> int {
+    HIP_TRY(hipEventRecord(bs->fork, st));
+    const int ns = std::min(count, kBatchStreams);
+    for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(bs->s[k], bs->fork, 0));
+    for (int i = 0; i < count; ++i) {
+      hipStream_t si = bs->s[i % kBatchStreams];
+      pdamd_problem* P = &probs[i];
+      char* w = ws + off[i];
+      if (P->patch_index_dev && counts[i] > 0)
+        HIP_TRY(hipMemcpyAsync(P->patch_index_dev, w + pls[i].off_idx, counts[i] * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, si));
+      const int r = dispatch_chunks(P, pls[i], w, counts[i], si, off_tm);
+      if (r) return r;
+    }
+    for (int k = 0; k < ns; ++k) {
+      HIP_TRY(hipEventRecord(bs->ev[k], bs->s[k]));
+      HIP_TRY(hipStreamWaitEvent(st, bs->ev[k], 0));
+    }
+    return PDAMD_OK;
+  };
+  if (!use_graph) return enqueue(st);
+  // graph key: every problem field that shapes the launches, and the counts
+  std::vector<char> key;
+  auto put = [&](const void* p, size_t b) {
+    const char* c = reinterpret_cast<const char*>(p);
+    key.insert(key.end(), c, c + b);
+  };
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  for (int i = 0; i < count; ++i) {
+    pdamd_problem q = probs[i];
+    q.stage_ms = nullptr;
+    q.stage_launches = nullptr;
+    q.selected = 0;
+    put(&q, sizeof q);
+    put(&counts[i], sizeof(int64_t));
+  }
+  put(&workspace_dev, sizeof workspace_dev);
+  auto& cache = graph_cache();
+  hipGraphExec_t exec = nullptr;
+  for (auto& e : cache)
+    if (e.dev == dev && e.key == key) exec = e.exec;
+  hipStream_t cs = bs->cap;
+  if (!exec) {
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    const int r = enqueue(cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (r) {
+      if (graph) hipGraphDestroy(graph);
+      return r;
+    }
+    if (ec != hipSuccess) return fail(PDAMD_EHIP, "graph capture: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(PDAMD_EHIP, "graph instantiate: %s", hipGetErrorString(ei));
+    if (cache.size() >= 8) {
+      hipGraphExecDestroy(cache.front().exec);
+      cache.erase(cache.begin());
+    }
+    cache.push_back({dev, key, exec});
+  }
+  // replay on the capture stream, ordered after the selection on the caller's
+  // stream and before whatever the caller enqueues next
+  HIP_TRY(hipEventRecord(bs->pre, st));
+  HIP_TRY(hipStreamWaitEvent(cs, bs->pre, 0));
+  HIP_TRY(hipGraphLaunch(exec, cs));
+  HIP_TRY(hipEventRecord(bs->done, cs));
+  HIP_TRY(hipStreamWaitEvent(st, bs->done, 0));
   return PDAMD_OK;
 }
 

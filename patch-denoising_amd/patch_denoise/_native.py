@@ -73,6 +73,8 @@ EXPORTS = (
     "pdamd_workspace_size",
     "pdamd_denoise",
     "pdamd_select_patches",
+    "pdamd_batch_workspace_size",
+    "pdamd_denoise_batch",
     "pdamd_backward_workspace_size",
     "pdamd_optimal_backward",
     "pdamd_last_error",
@@ -103,6 +105,11 @@ def load_library():
         ctypes.c_void_p,
     ]
     lib.pdamd_select_patches.restype = ctypes.c_int
+    lib.pdamd_batch_workspace_size.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+    lib.pdamd_batch_workspace_size.restype = ctypes.c_int
+    lib.pdamd_denoise_batch.argtypes = [P, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.c_int, ctypes.c_void_p]
+    lib.pdamd_denoise_batch.restype = ctypes.c_int
     lib.pdamd_backward_workspace_size.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
     lib.pdamd_backward_workspace_size.restype = ctypes.c_int
     lib.pdamd_optimal_backward.argtypes = [
@@ -178,7 +185,8 @@ class DeviceRun:
     :meth:`result` copies them back.
     """
 
-    def __init__(self, prob, data, mask=None, var_map=None, want_patch_info=False):
+    def __init__(self, prob, data, mask=None, var_map=None, want_patch_info=False,
+                 workspace=True):
         torch = _torch()
         self.torch = torch
         self.prob = prob
@@ -222,8 +230,10 @@ class DeviceRun:
             prob.rank_dev = self.info["rank"].data_ptr()
             prob.noise_dev = self.info["noise"].data_ptr()
             prob.eig_dev = self.info["eig"].data_ptr()
-        self.ws_bytes = workspace_bytes(prob)
-        self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
+        self.ws_bytes, self.ws = 0, None
+        if workspace:
+            self.ws_bytes = workspace_bytes(prob)
+            self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
 
     def run(self):
         stream = self.torch.cuda.current_stream().cuda_stream
@@ -272,3 +282,41 @@ def select_patches(prob, mask=None):
         ctypes.byref(prob), idx.data_ptr(), ctypes.byref(cnt), ws.data_ptr(), nb,
         torch.cuda.current_stream().cuda_stream))
     return idx[: cnt.value].cpu().numpy()
+
+
+class BatchRun:
+    """Several volumes in one ``pdamd_denoise_batch`` call.
+
+    ``items``: list of ``(prob, data, mask, var_map)`` as for :class:`DeviceRun`
+    (each problem already configured).  Device buffers are allocated once and
+    reused by every :meth:`run`, so repeated runs can replay the captured graph.
+    """
+
+    def __init__(self, items, use_graph=True):
+        torch = _torch()
+        self.torch = torch
+        self.runs = [DeviceRun(prob, data, mask, var_map, workspace=False)
+                     for prob, data, mask, var_map in items]
+        n = len(self.runs)
+        self.probs = (Problem * n)()
+        for i, r in enumerate(self.runs):
+            ctypes.memmove(ctypes.byref(self.probs[i]), ctypes.byref(r.prob), ctypes.sizeof(Problem))
+        nb = ctypes.c_size_t(0)
+        _check(load_library().pdamd_batch_workspace_size(self.probs, n, ctypes.byref(nb)))
+        self.ws_bytes = int(nb.value)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
+        self.use_graph = bool(use_graph)
+
+    def run(self):
+        stream = self.torch.cuda.current_stream().cuda_stream
+        _check(load_library().pdamd_denoise_batch(self.probs, len(self.runs), self.ws.data_ptr(),
+                                                  self.ws_bytes, int(self.use_graph), stream))
+        return self
+
+    @property
+    def selected(self):
+        return [int(p.selected) for p in self.probs]
+
+    def results(self, shapes):
+        return [r.result(s) for r, s in zip(self.runs, shapes)]

@@ -116,6 +116,28 @@ def _geometry(data_shape, p_s, p_o):
     return shape4, patch4, step4
 
 
+# Stack of job lists: while non-empty, denoise() records its prepared problem
+# in the innermost list instead of running it (patch_denoise.batch).
+_COLLECT = []
+
+
+class _Deferred:
+    """A configured denoise() call waiting for a batched run."""
+
+    def __init__(self, denoiser, prob, data, mask_arr, var_map, p_s, p_o, progbar):
+        self.denoiser, self.prob, self.data = denoiser, prob, data
+        self.mask_arr, self.var_map, self.p_s, self.p_o = mask_arr, var_map, p_s, p_o
+        self.progbar = progbar
+
+    def finish(self, out, w, n_selected):
+        """The 4-tuple denoise() would have returned."""
+        self.denoiser.n_selected = n_selected
+        _progress(self.progbar, n_selected)
+        shape = self.data.shape
+        return (out, w, PatchedArray(shape, self.p_s, self.p_o, dtype=np.float32),
+                PatchedArray(shape, self.p_s, self.p_o, dtype=np.int32))
+
+
 class BaseSpaceTimeDenoiser(abc.ABC):
     """Patch-based low-rank denoiser for space x time data (base.py:154-384).
 
@@ -171,6 +193,10 @@ class BaseSpaceTimeDenoiser(abc.ABC):
         N.dtype_code(input_data.dtype)
         prob, p_s, p_o, mask_arr = self._prepare(input_data, mask, mask_threshold)
         var_map = self._configure(prob, input_data, p_s)
+        if _COLLECT:  # inside patch_denoise.batch.denoise_batch: run later, batched
+            job = _Deferred(self, prob, input_data, mask_arr, var_map, p_s, p_o, progbar)
+            _COLLECT[-1].append(job)
+            return job
         run = N.DeviceRun(prob, input_data, mask_arr, var_map).run()
         out, w = run.result(input_data.shape)
         self.n_selected = run.selected

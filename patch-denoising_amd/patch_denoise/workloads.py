@@ -157,6 +157,30 @@ def config4(seed=4, n_volumes=8, scale=1.0):
     return vols
 
 
+def config4_calls(vols, max_n=128):
+    """Functional-API calls for the batched workload (``config4()`` volumes):
+    mp_pca / hybrid_pca (noise_std = the sigma map) / optimal_thresholding /
+    raw_svt (threshold = tau), patch p, overlap p - 1, the volume's mask.
+    Volumes whose Gram size min(p^3, T) exceeds ``max_n`` are returned in
+    ``skipped`` (index, reason) instead."""
+    calls, skipped = [], []
+    names = {"mppca": "mp_pca", "hybrid": "hybrid_pca", "optimal": "optimal_thresholding",
+             "raw": "raw_svt"}
+    for i, v in enumerate(vols):
+        p, T = v["patch"], v["data"].shape[-1]
+        n = min(p**3, T)
+        if n > max_n:
+            skipped.append((i, f"Gram size min({p}^3, {T}) = {n} > {max_n}"))
+            continue
+        kw = {"mask": v["mask"]}
+        if v["method"] == "hybrid":
+            kw["noise_std"] = v["sigma_map"]
+        elif v["method"] == "raw":
+            kw["threshold"] = float(v["tau"])
+        calls.append((names[v["method"]], (v["data"], p, p - 1), kw))
+    return calls, skipped
+
+
 def golden_volumes():
     """Tiny seeded volumes for reference fixtures (each runs in seconds on CPU)."""
     rng = np.random.default_rng(11)
