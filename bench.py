@@ -174,6 +174,34 @@ def bench_batched(steps):
         speedup_graph_vs_sequential=t_seq / t_graph)
 
 
+def bench_noise_map(steps):
+    """Sliding-window noise map (estimate_noise) on an 80x80x48x300 float32
+    volume, block 3, input resident in HBM.  HBM-bound: the algorithmic bytes
+    are one read of the volume plus the per-voxel moments and the map."""
+    import torch
+
+    from patch_denoise import _native as N
+
+    g = torch.Generator(device="cuda").manual_seed(8)
+    x = 800.0 + 30.0 * torch.randn((80, 80, 48, 300), generator=g, device="cuda",
+                                   dtype=torch.float32)
+    N.estimate_noise(x, 3)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        N.estimate_noise(x, 3)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nvox = 80 * 80 * 48
+    algo = x.numel() * 4 + nvox * (3 * 8 * 2 + 8)
+    return dict(workload="noise_map_80x80x48x300_block3", ms_per_call=ms,
+                algorithmic_bytes=algo, achieved_GBs=algo / (ms / 1e3) / 1e9,
+                peak_GBs=HBM_PEAK_GBS, frac=algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                note="time includes the two kernels and the per-call buffer setup")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +214,8 @@ def main():
                     help="steps of the batched multi-volume measurement; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")
+    ap.add_argument("--only", choices=["all", "headline", "differentiable", "batched", "noise"],
+                    default="all", help="run only one part (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -201,6 +231,15 @@ def main():
     from patch_denoise.space_time.base import _geometry
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.only == "differentiable":
+        print(json.dumps({"differentiable": bench_differentiable(max(args.diff_steps, 1))}))
+        return
+    if args.only == "batched":
+        print(json.dumps({"batched": bench_batched(max(args.batch_steps, 1))}))
+        return
+    if args.only == "noise":
+        print(json.dumps({"noise_map": bench_noise_map(5)}))
+        return
     t_gen = time.perf_counter()
     data, mask = workloads.config2()
     gen_s = time.perf_counter() - t_gen
@@ -289,8 +328,10 @@ def main():
                    per_patch_ms=cb["per_patch_s"] * 1e3, blas_threads=cb["blas"],
                    host_cpus=os.cpu_count())
 
-    diff = bench_differentiable(args.diff_steps) if args.diff_steps > 0 else None
-    batched = bench_batched(args.batch_steps) if args.batch_steps > 0 else None
+    full = args.only == "all"
+    diff = bench_differentiable(args.diff_steps) if full and args.diff_steps > 0 else None
+    batched = bench_batched(args.batch_steps) if full and args.batch_steps > 0 else None
+    noise_map = bench_noise_map(5) if full else None
 
     line = {
         "metric": "MP-PCA voxel-frames/s at 80x80x48x300",
@@ -314,6 +355,7 @@ def main():
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "differentiable": diff,
         "batched": batched,
+        "noise_map": noise_map,
         "datagen_s": gen_s,
     }
     print(json.dumps(line))

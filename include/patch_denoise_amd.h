@@ -182,6 +182,19 @@ int pdamd_optimal_backward(pdamd_problem* prob, const void* grad_out_dev,
                            const float* weights_dev, void* grad_in_dev, double* grad_lam_dev,
                            void* workspace_dev, size_t workspace_bytes, void* stream);
 
+/* Sliding-window noise map, replaces space_time/utils.py:207-242
+   (estimate_noise): out[c] = population standard deviation of the
+   block_size^d x T elements of the stride-1 window centred at voxel c, for
+   every window inside the volume; other voxels of out are left untouched
+   (the reference leaves them uninitialised).  in_dev: shape4 = (X, Y, Z, T)
+   with leading spatial sizes 1 when nspatial < 3, dtype PDAMD_F32 / F64 /
+   C64 / C128; out_dev: float64 X x Y x Z.  Every one of the nspatial spatial
+   sizes must exceed block_size (the reference raises otherwise).          */
+int pdamd_noise_workspace_size(const int64_t* shape4, size_t* bytes);
+int pdamd_estimate_noise(const void* in_dev, int dtype, const int64_t* shape4, int nspatial,
+                         int block_size, double* out_dev, void* workspace_dev,
+                         size_t workspace_bytes, void* stream);
+
 const char* pdamd_last_error(void);
 const char* pdamd_version(void);
 

@@ -25,6 +25,8 @@ Pinning (checked in ``tests/test_oracle.py``):
 
 import logging
 
+import itertools
+
 import numpy as np
 from scipy.integrate import quad
 from scipy.linalg import eigh, svd
@@ -439,3 +441,31 @@ def optimal_thresholding(input_data, patch_shape, patch_overlap, mask=None,
         input_data, ps, po,
         lambda X, i: optimal_patch(X, shrink, mpm, None if prior is None else prior.get(i)),
         recombination, mask, mask_threshold)
+
+
+# --------------------------------------------------------------------------
+# Sliding-window noise map (space_time/utils.py:207-242)
+# --------------------------------------------------------------------------
+
+
+def estimate_noise(noise_sequence, block_size=1):
+    """Population std of each stride-1 block^d x T window, written at the
+    window centre (utils.py:207-242, corners from get_patch_locs, utils.py:8-55).
+
+    Window corners run 0..S-b in every spatial dimension; like the reference
+    (whose corner list breaks with an IndexError), a spatial size <= b raises
+    IndexError.  Voxels that are no window's centre are NaN here (the
+    reference leaves them uninitialised).
+    """
+    vshape = noise_sequence.shape[:-1]
+    b = int(block_size)
+    corners = []
+    for s in vshape:
+        if s - b <= 0:
+            raise IndexError("index -1 is out of bounds for axis 0 with size 0")
+        corners.append(range(0, s - b + 1))
+    out = np.full(vshape, np.nan)
+    for c in itertools.product(*corners):
+        win = tuple(slice(ci, ci + b) for ci in c)
+        out[tuple(ci + b // 2 for ci in c)] = np.std(noise_sequence[win])
+    return out

@@ -1936,11 +1936,11 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
     pl->kmax = n;
   }
   pl->kb = std::min(kKB, pl->kmax);
-  // chunk: keep the per-chunk buffers under ~1 GiB (4 GiB for the backward)
+  // chunk: keep the per-chunk buffers under ~4 GiB (6 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
   if (bwd) per += (3 * (size_t)M * n + (size_t)n * n + 2 * (size_t)n) * pl->ebytes + 8;
-  int64_t chunk = (int64_t)((bwd ? (4ull << 30) : (1ull << 30)) / per);
+  int64_t chunk = (int64_t)((bwd ? (6ull << 30) : (4ull << 30)) / per);
   chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
   pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
   // workspace layout
@@ -2389,6 +2389,10 @@ std::vector<GraphEntry>& graph_cache() {
 }
 
 }  // namespace
+
+namespace pdamd_detail {
+void set_error(const char* msg) { g_err = msg; }
+}  // namespace pdamd_detail
 
 extern "C" {
 

@@ -77,6 +77,8 @@ EXPORTS = (
     "pdamd_denoise_batch",
     "pdamd_backward_workspace_size",
     "pdamd_optimal_backward",
+    "pdamd_noise_workspace_size",
+    "pdamd_estimate_noise",
     "pdamd_last_error",
     "pdamd_version",
 )
@@ -117,6 +119,13 @@ def load_library():
         ctypes.c_size_t, ctypes.c_void_p,
     ]
     lib.pdamd_optimal_backward.restype = ctypes.c_int
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    lib.pdamd_noise_workspace_size.argtypes = [I64P, ctypes.POINTER(ctypes.c_size_t)]
+    lib.pdamd_noise_workspace_size.restype = ctypes.c_int
+    lib.pdamd_estimate_noise.argtypes = [ctypes.c_void_p, ctypes.c_int, I64P, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_void_p]
+    lib.pdamd_estimate_noise.restype = ctypes.c_int
     lib.pdamd_last_error.restype = ctypes.c_char_p
     lib.pdamd_version.restype = ctypes.c_char_p
     _lib = lib
@@ -320,3 +329,35 @@ class BatchRun:
 
     def results(self, shapes):
         return [r.result(s) for r, s in zip(self.runs, shapes)]
+
+
+def estimate_noise(data, block_size):
+    """Sliding-window noise map on the GPU (``pdamd_estimate_noise``).
+
+    ``data``: NumPy array or CUDA tensor with 1-3 spatial axes then time.
+    Returns a float64 CUDA tensor of the spatial shape, NaN where no window
+    is centred.
+    """
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    shape = tuple(int(v) for v in data.shape)
+    nsp = len(shape) - 1
+    if not 1 <= nsp <= 3:
+        raise ValueError("estimate_noise takes 1 to 3 spatial axes plus time")
+    code = dtype_code(data.dtype if isinstance(data, np.ndarray) else
+                      np.dtype(str(data.dtype).replace("torch.", "")))
+    shape4 = (1,) * (3 - nsp) + shape
+    if isinstance(data, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(data)).to(dev)
+    else:
+        x = data.contiguous()
+    out = torch.full(shape4[:3], float("nan"), dtype=torch.float64, device=dev)
+    s4 = (ctypes.c_int64 * 4)(*shape4)
+    nb = ctypes.c_size_t(0)
+    lib = load_library()
+    _check(lib.pdamd_noise_workspace_size(s4, ctypes.byref(nb)))
+    ws = torch.empty(max(int(nb.value), 8), dtype=torch.uint8, device=dev)
+    _check(lib.pdamd_estimate_noise(x.data_ptr(), code, s4, nsp, int(block_size), out.data_ptr(),
+                                    ws.data_ptr(), int(nb.value),
+                                    torch.cuda.current_stream().cuda_stream))
+    return out.reshape(shape[:-1])

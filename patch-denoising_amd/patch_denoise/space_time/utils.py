@@ -44,3 +44,31 @@ def marchenko_pastur_median(beta, eps=1e-7):
         if not moved:
             break
     return (lo + hi) / 2
+
+
+def estimate_noise(noise_sequence, block_size=1):
+    """Estimate a noise map from a noise-only sequence (reference:
+    space_time/utils.py:207-242), on the GPU.
+
+    Each voxel that is the centre of a stride-1 window of ``block_size`` voxels
+    per spatial axis gets the standard deviation (ddof 0) of the window's
+    voxels over all frames.  Other voxels are NaN (the reference leaves them
+    uninitialised).  A spatial size <= ``block_size`` raises IndexError, as the
+    reference's patch-corner list does.
+
+    Parameters
+    ----------
+    noise_sequence : np.ndarray of shape (X, Y, Z, T) (1 to 3 spatial axes)
+    block_size : int
+
+    Returns
+    -------
+    np.ndarray of the spatial shape, float64
+    """
+    from .. import _native as N
+
+    noise_sequence = np.asarray(noise_sequence)
+    for s in noise_sequence.shape[:-1]:
+        if s - block_size <= 0:
+            raise IndexError("index -1 is out of bounds for axis 0 with size 0")
+    return N.estimate_noise(noise_sequence, block_size).cpu().numpy()

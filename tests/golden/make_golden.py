@@ -20,6 +20,8 @@ Fixtures written (all small, ``np.savez_compressed``):
 * ``nordic_tau.npz``   NORDIC Monte-Carlo threshold for seeded global state
 * ``cfg1_mppca.npz``   full 32x32x16x50 MP-PCA run: per-patch rank/variance
                        and a seeded sample of output voxel-frames
+* ``noise_map.npz``    sliding-window noise maps (estimate_noise) on small
+                       1-D / 2-D / 3-D, float / complex volumes
 """
 
 import os
@@ -192,11 +194,40 @@ def cfg1_full():
     )
 
 
+def noise_map():
+    """Sliding-window noise map (space_time/utils.py:207-242) on small volumes.
+
+    The reference leaves voxels that are no window's centre uninitialised
+    (np.empty); those entries are stored as NaN, the rest are its output.
+    """
+    rng = np.random.default_rng(21)
+    cases = {
+        "f32_b1": (rng.standard_normal((7, 6, 5, 12)) * 3 + 10).astype(np.float32),
+        "f32_b2": (rng.standard_normal((7, 6, 5, 12)) * 3 + 10).astype(np.float32),
+        "f32_b3": (rng.standard_normal((7, 6, 5, 12)) * 3 + 10).astype(np.float32),
+        "f64_2d_b3": rng.standard_normal((6, 7, 9)) * 2 - 1,
+        "c64_b2": (rng.standard_normal((5, 6, 4, 10)) + 1j * rng.standard_normal((5, 6, 4, 10))
+                   ).astype(np.complex64),
+        "f32_1d_b3": (rng.standard_normal((12, 8)) * 5).astype(np.float32),
+    }
+    blocks = {"f32_b1": 1, "f32_b2": 2, "f32_b3": 3, "f64_2d_b3": 3, "c64_b2": 2, "f32_1d_b3": 3}
+    arrays = {}
+    for k, data in cases.items():
+        b = blocks[k]
+        out = RU.estimate_noise(data, block_size=b)
+        written = np.zeros(data.shape[:-1], bool)
+        written[tuple(slice(b // 2, s - b + b // 2 + 1) for s in data.shape[:-1])] = True
+        arrays[k + "_in"] = data
+        arrays[k + "_out"] = np.where(written, out, np.nan)
+        arrays[k + "_block"] = np.asarray(b)
+    _save("noise_map.npz", **arrays)
+
+
 if __name__ == "__main__":
     import warnings
 
     warnings.simplefilter("ignore")
-    which = sys.argv[1:] or ["e2e", "patches", "mpm", "tau", "cfg1"]
+    which = sys.argv[1:] or ["e2e", "patches", "mpm", "tau", "cfg1", "noise"]
     if "mpm" in which:
         mp_median_grid()
     if "tau" in which:
@@ -207,3 +238,5 @@ if __name__ == "__main__":
         e2e_cases()
     if "cfg1" in which:
         cfg1_full()
+    if "noise" in which:
+        noise_map()
