@@ -118,8 +118,10 @@ def nordic_threshold(n_rows, n_frames, n_iter):
     ``np.random`` stream (reference: lowrank.py:221-232).
 
     Small draws use the same ``scipy.linalg.svd`` call as the reference; large
-    ones take the largest eigenvalue of the float64 Gram (same value to
-    ~1e-15 relative, a fraction of the cost).
+    ones take the largest eigenvalue of the float64 Gram ``a^T a`` (same value
+    to ~1e-15 relative), with the Gram formed on the GPU (a plain FP64 library
+    GEMM) when one is present.  The draws themselves stay on the host: they
+    must be the global stream's own legacy Gaussian sequence.
     """
     total = 0.0
     for _ in range(n_iter):
@@ -127,9 +129,21 @@ def nordic_threshold(n_rows, n_frames, n_iter):
         if a.size <= 4_000_000:
             total = total + max(svd(a, compute_uv=False))
         else:
-            g = a.T @ a
+            g = _gram_f64(a)
             total = total + np.sqrt(eigvalsh(g, subset_by_index=(n_frames - 1, n_frames - 1))[0])
     return np.float64(total) / n_iter
+
+
+def _gram_f64(a):
+    """a^T a in float64, on the GPU when available."""
+    try:
+        import torch
+    except ImportError:  # pragma: no cover - torch is part of the runtime
+        torch = None
+    if torch is not None and torch.cuda.is_available():
+        d = torch.from_numpy(a).cuda()
+        return (d.T @ d).cpu().numpy()
+    return a.T @ a
 
 
 class NordicDenoiser(RawSVDDenoiser):

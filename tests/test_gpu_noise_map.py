@@ -47,3 +47,17 @@ def test_gpu_noise_map_80x80x48x300():
     ref = O.estimate_noise(sub, 3)[1:-1, 1:-1, 1:-1]
     got = out[lo[0] + 1:lo[0] + 5, lo[1] + 1:lo[1] + 5, lo[2] + 1:lo[2] + 5]
     np.testing.assert_allclose(got, ref, rtol=1e-5)
+
+
+def test_nordic_threshold_large_draws_on_gpu():
+    """NORDIC Monte-Carlo threshold with draws above the host-SVD size (GPU
+    Gram path) against the oracle's svd of the same seeded draws."""
+    from oracle import cpu_ref as O
+    from patch_denoise.space_time.lowrank import nordic_threshold
+
+    np.random.seed(5)
+    a = nordic_threshold(125 * 200, 200, 3)  # 5e6 elements per draw
+    np.random.seed(5)
+    b = O.nordic_threshold((125 * 200,), 200, 1.0, 3)
+    print(f"[noise] nordic tau gpu-gram {a!r} oracle {b!r}", flush=True)
+    assert a == pytest.approx(b, rel=1e-13)
