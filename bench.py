@@ -77,6 +77,53 @@ def cpu_baseline(data, mask, ps, po, sel, stride):
                 full_estimate_s=full, blas=blas)
 
 
+def plain_gpu_baseline(data, mask_unused, sel, stride, M=125):
+    """The plain one-patch-per-launch FP32 GPU path: per selected patch, a
+    chain of PyTorch launches (gather, centre, Gram matmul, torch.linalg.eigh,
+    MP-PCA rule, projection, scatter-add) with no batching.  Timed on every
+    ``stride``-th selected patch and extrapolated to all of them."""
+    import torch
+
+    x = torch.from_numpy(data).cuda()
+    X, Y, Z, T = data.shape
+    out = torch.zeros_like(x)
+    w = torch.zeros(x.shape, device="cuda")
+    gy, gz = Y - 4, Z - 4
+    sample = sel[::stride]
+
+    def one(L):
+        i, j, k = L // (gy * gz), (L // gz) % gy, L % gz
+        P = x[i:i + 5, j:j + 5, k:k + 5].reshape(M, T)
+        m = P.mean(0)
+        Xc = P - m
+        ev, V = torch.linalg.eigh(Xc @ Xc.T / (M - 1))
+        top = ev[-10:].flip(0)
+        kk = 0
+        mv = top.mean() * (4.0 * ((10 - kk + 1) / M) ** 0.5)
+        while kk < 10 and bool(mv < (top[kk] - top[-1])):
+            kk += 1
+            mv = top[kk:].mean() * (4.0 * ((T - kk + 1) / M) ** 0.5)
+        var = top[kk:].mean()
+        r = int((top > var).sum())
+        Vr = V[:, M - r:]
+        Pd = Vr @ (Vr.T @ Xc) + m
+        th = 1.0 / (2 + r)
+        out[i:i + 5, j:j + 5, k:k + 5] += th * Pd.reshape(5, 5, 5, T)
+        w[i:i + 5, j:j + 5, k:k + 5] += th
+
+    for L in sample[:20]:
+        one(int(L))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for L in sample:
+        one(int(L))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    per = dt / len(sample)
+    return dict(per_patch_ms=per * 1e3, sample_patches=int(len(sample)), seconds=dt,
+                full_estimate_s=per * len(sel))
+
+
 def bench_differentiable(steps):
     """Forward and forward+backward of the differentiable optimal-shrinkage
     path on the rank-6 64x64x32x120 workload (patch 5, overlap 4, Frobenius
@@ -209,6 +256,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain one-patch-per-launch GPU baseline")
+    ap.add_argument("--plain-stride", type=int, default=64)
     ap.add_argument("--precision", default="auto")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="steps of the batched multi-volume measurement; 0 skips it")
@@ -333,6 +383,39 @@ def main():
     batched = bench_batched(args.batch_steps) if full and args.batch_steps > 0 else None
     noise_map = bench_noise_map(5) if full else None
 
+    split = None
+    if args.only == "all" and args.precision == "auto":
+        # the same workload with the split-BF16 three-product Gram
+        pb = N.make_problem(s4, p4, st4, np.float32, N.MPPCA, "weighted", "bf16x3")
+        pb.threshold_scale2 = 1.0
+        pb.scale_f64 = 0
+        pb.mask_kind = N.MASK_SPATIAL
+        pb.mask_threshold = 50.0
+        rb = N.DeviceRun(pb, dev_x, mask, None)
+        rb.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rb.run()
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / args.steps
+        mk = torch.from_numpy(mask).cuda()[..., None].expand_as(run.out) & torch.isfinite(run.out)
+        rel_bf = float((rb.out - run.out)[mk].double().norm() / run.out[mk].double().norm())
+        split = dict(precision="bf16x3", ms_per_step=tb * 1e3, value=vf / tb,
+                     rel_l2_vs_fp32=rel_bf, speedup_vs_fp32=(ms_per_step / 1e3) / tb)
+        del rb
+
+    plain = None
+    if not args.no_plain:
+        pg = plain_gpu_baseline(data, mask, sel.astype(np.int64), args.plain_stride)
+        plain = dict(value=vf / pg["full_estimate_s"], unit="voxel-frames/s",
+                     sample=(f"every {args.plain_stride}th selected patch "
+                             f"({pg['sample_patches']} patches, {pg['seconds']:.1f} s), "
+                             "extrapolated to all selected patches"),
+                     per_patch_ms=pg["per_patch_ms"],
+                     path="one patch per launch chain: torch gather, Gram matmul, "
+                          "torch.linalg.eigh (FP32), MP-PCA rule, projection, scatter-add")
+
     line = {
         "metric": "MP-PCA voxel-frames/s at 80x80x48x300",
         "value": value,
@@ -353,12 +436,23 @@ def main():
         "cpu_baseline": cpu,
         "stages": stages,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "split_bf16": split,
+        "plain_gpu_baseline": plain,
+        "speedup_vs_plain_gpu": (value / plain["value"]) if plain else None,
         "differentiable": diff,
         "batched": batched,
         "noise_map": noise_map,
         "datagen_s": gen_s,
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=_jsonable))
+
+
+def _jsonable(o):
+    """0-d tensors / NumPy scalars in the result line become Python numbers."""
+    if hasattr(o, "item"):
+        print(f"bench: converted a {type(o).__name__} in the JSON line", file=sys.stderr)
+        return o.item()
+    raise TypeError(f"{type(o).__name__} is not JSON serialisable")
 
 
 if __name__ == "__main__":

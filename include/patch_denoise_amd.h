@@ -55,7 +55,10 @@ enum { PDAMD_LOSS_FRO = 0, PDAMD_LOSS_NUC = 1, PDAMD_LOSS_OPE = 2 };
 enum { PDAMD_MASK_NONE = 0, PDAMD_MASK_SPATIAL = 1, PDAMD_MASK_FULL = 2 };
 
 /* eigen-stage precision */
-enum { PDAMD_PREC_AUTO = 0, PDAMD_PREC_FP32 = 1, PDAMD_PREC_FP64 = 2 };
+/* PDAMD_PREC_BF16X3: FP32 eigen stage with the Gram (FP32 data, fewer
+   voxels than frames, Gram size 65..128) from split-BF16 three-product MFMA
+   with FP32 accumulation; other shapes use the FP32 Gram.                   */
+enum { PDAMD_PREC_AUTO = 0, PDAMD_PREC_FP32 = 1, PDAMD_PREC_FP64 = 2, PDAMD_PREC_BF16X3 = 3 };
 
 /* error codes */
 enum {

@@ -385,7 +385,10 @@ template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
   return {a.x / (R)n, a.y / (R)n};
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kBfW = 32 + 8;  // bf16 staging row: SC frames + pad (80 B, 16-B aligned)
+
+template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
   using R = typename real_of<E>::type;
@@ -400,6 +403,10 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   // FP32 real Grams of size 65..128 use MFMA 32x32x2: wave w owns the 64x64
   // block (w >> 1, w & 1), i.e. 2x2 tiles of 32x32 (16 accumulators per lane each).
   constexpr bool MF = std::is_same<E, float>::value && TILE == 8;
+  // Split-BF16 (M-side): each centred value x = hi + lo with hi = bf16(x),
+  // lo = bf16(x - hi); G += hi hi^T + hi lo^T + lo hi^T on MFMA 32x32x16 bf16
+  // with FP32 accumulation (the lo lo^T term, ~2^-16 relative, is dropped).
+  constexpr bool BFM = BF && MF && !TSIDE && SC == 32;
   const int wv = tid / kWave, lane = tid % kWave;
   const int bi = wv >> 1, bj = wv & 1, lhi = lane >> 5, llo = lane & 31;
 
@@ -409,6 +416,8 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   E* Sc = SAME ? reinterpret_cast<E*>(Sraw) : lds_carve<E>(cur, (size_t)SC * LDS_W);
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? g.M : NB);  // row offsets of the patch
+  __bf16* Sh = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;  // Sh[i][k]
+  __bf16* Sl = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     int64_t sp;
@@ -434,7 +443,27 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
     }
     // one reduction index k of the staged S[k][i] (i = Gram row/column index)
     auto accum = [&](const E* Sk, int kcount) {
-      if constexpr (MF) {
+      if constexpr (BFM) {
+        // lane: rows 64 b + 32 h + llo, frames k + 8 lhi .. + 7 (one 16-B load each)
+        auto ld = [&](const __bf16* S, int row, int k) {
+          return *reinterpret_cast<const bf16x8*>(S + row * kBfW + k + 8 * lhi);
+        };
+#pragma unroll
+        for (int k = 0; k < SC; k += 16) {  // frames >= kcount are zero in Sh / Sl
+          const bf16x8 ha[2] = {ld(Sh, 64 * bi + llo, k), ld(Sh, 64 * bi + 32 + llo, k)};
+          const bf16x8 la[2] = {ld(Sl, 64 * bi + llo, k), ld(Sl, 64 * bi + 32 + llo, k)};
+          const bf16x8 hb[2] = {ld(Sh, 64 * bj + llo, k), ld(Sh, 64 * bj + 32 + llo, k)};
+          const bf16x8 lb[2] = {ld(Sl, 64 * bj + llo, k), ld(Sl, 64 * bj + 32 + llo, k)};
+#pragma unroll
+          for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+            for (int b2 = 0; b2 < 2; ++b2) {
+              mc[a2][b2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha[a2], hb[b2], mc[a2][b2], 0, 0, 0);
+              mc[a2][b2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha[a2], lb[b2], mc[a2][b2], 0, 0, 0);
+              mc[a2][b2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la[a2], hb[b2], mc[a2][b2], 0, 0, 0);
+            }
+        }
+      } else if constexpr (MF) {
         for (int k = 0; k < kcount; k += 2) {
           const float* row = reinterpret_cast<const float*>(Sk) + (k + lhi) * LDS_W;
           const float a0 = row[64 * bi + llo], a1 = row[64 * bi + 32 + llo];
@@ -479,7 +508,14 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
         for (int q = tid; q < SC * NB; q += kNT) {
           const int t = q / NB, r = q % NB;
           const IN v = Sraw[t * LDS_W + r];
-          Sc[t * LDS_W + r] = (t < tc && r < M) ? cvt<E>(v - mean[t]) : E(R(0));
+          const E x = (t < tc && r < M) ? cvt<E>(v - mean[t]) : E(R(0));
+          if constexpr (BFM) {
+            const __bf16 hi = (__bf16)x;
+            Sh[r * kBfW + t] = hi;
+            Sl[r * kBfW + t] = (__bf16)(x - (float)hi);
+          } else {
+            Sc[t * LDS_W + r] = x;
+          }
         }
         lds_barrier();
         if constexpr (MF) {
@@ -602,7 +638,7 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   }
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
+template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 size_t gram_lds_bytes(int n, int M) {
   constexpr int NB = 16 * TILE;
   constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
@@ -612,6 +648,8 @@ size_t gram_lds_bytes(int n, int M) {
   size_t s = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? (size_t)M : (size_t)NB) * sizeof(int64_t));
+  if (BF && std::is_same<E, float>::value && TILE == 8 && !TSIDE && SC == 32)
+    s += 2 * al((size_t)NB * kBfW * 2);
   return s;
 }
 
@@ -1859,6 +1897,7 @@ struct Plan {
   bool tside;
   int etype;  // 0 float, 1 double, 2 cpx<float>, 3 cpx<double>
   int rule_f64;
+  bool bf16x3 = false;  // split-BF16 Gram (FP32 data, M-side MFMA path)
   int64_t chunk;
   size_t ebytes, rbytes;
   // workspace offsets
@@ -1916,13 +1955,17 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   const bool in64 = P->dtype == PDAMD_F64 || P->dtype == PDAMD_C128;
   pl->rule_f64 = in64 ? 1 : 0;
   bool e64;
-  if (P->precision == PDAMD_PREC_FP32) e64 = in64;  // never below the input precision
+  if (P->precision < 0 || P->precision > PDAMD_PREC_BF16X3)
+    return fail(PDAMD_EINVAL, "bad precision %d", P->precision);
+  if (P->precision == PDAMD_PREC_FP32 || P->precision == PDAMD_PREC_BF16X3)
+    e64 = in64;  // never below the input precision
   else if (P->precision == PDAMD_PREC_FP64) e64 = true;
   else e64 = in64 || (P->method == PDAMD_HARD || P->method == PDAMD_OPTIMAL);
   if (cplx && n > 64 && e64 && !in64) {
     // complex double Gram of n > 64 does not fit LDS with this kernel
   }
   pl->etype = (cplx ? 2 : 0) + (e64 ? 1 : 0);
+  pl->bf16x3 = P->precision == PDAMD_PREC_BF16X3 && !e64 && !cplx;
   pl->rbytes = e64 ? 8 : 4;
   pl->ebytes = pl->rbytes * (cplx ? 2 : 1);
   pl->kmax = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? std::min(10, n) : n;
@@ -2046,11 +2089,11 @@ int grid_for(int64_t np, int per_cu) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(np, (int64_t)n_cu * per_cu));
 }
 
-template <class IN, class E, bool TSIDE, int TILE>
+template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE>(pl.n, pl.g.M);
+  const size_t lds = gram_lds_bytes<IN, E, TSIDE, TILE, BF>(pl.n, pl.g.M);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = gram_kernel<IN, E, TSIDE, TILE>;
+  auto kern = gram_kernel<IN, E, TSIDE, TILE, BF>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
@@ -2076,7 +2119,10 @@ int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t s
   switch (pl.tile) {
     case 2: return launch_gram<IN, E, TSIDE, 2>(pl, vol, ga, st);
     case 4: return launch_gram<IN, E, TSIDE, 4>(pl, vol, ga, st);
-    default: return launch_gram<IN, E, TSIDE, 8>(pl, vol, ga, st);
+    default:
+      if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value && !TSIDE)
+        if (pl.bf16x3) return launch_gram<IN, E, TSIDE, 8, true>(pl, vol, ga, st);
+      return launch_gram<IN, E, TSIDE, 8>(pl, vol, ga, st);
   }
 }
 

@@ -20,7 +20,7 @@ WEIGHTED, AVERAGE, CENTER = 0, 1, 2
 LOSS = {"fro": 0, "nuc": 1, "ope": 2}
 RECOMB = {"weighted": WEIGHTED, "average": AVERAGE, "center": CENTER}
 MASK_NONE, MASK_SPATIAL, MASK_FULL = 0, 1, 2
-PREC = {"auto": 0, "fp32": 1, "fp64": 2}
+PREC = {"auto": 0, "fp32": 1, "fp64": 2, "bf16x3": 3}
 
 _DTYPES = {
     np.dtype(np.float32): F32,

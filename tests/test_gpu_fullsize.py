@@ -153,3 +153,34 @@ def test_nordic_complex_window():
     e = rel_l2(win, ref)
     print(f"[fullsize] window rel-L2 out {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+def test_split_bf16_gram_mppca():
+    """Split-BF16 three-product Gram (precision="bf16x3"): the 80x80x48x300
+    MP-PCA output within 1e-3 relative L2 of the FP32 path (and of the oracle
+    on a window); ranks equal except near-ties."""
+    from oracle import cpu_ref as O
+    from patch_denoise import MPPCADenoiser, workloads
+
+    data, mask = workloads.config2()
+    ps, po = (5, 5, 5, 300), (4, 4, 4, 300)
+    print("[fullsize] mppca 80x80x48x300 fp32 vs bf16x3", flush=True)
+    out32, w32, info32 = _run_gpu(MPPCADenoiser(5, 4, threshold_scale=1.0, precision="fp32"), data, mask)
+    outbf, wbf, infobf = _run_gpu(MPPCADenoiser(5, 4, threshold_scale=1.0, precision="bf16x3"), data,
+                                  mask)
+    m = mask[..., None] & np.isfinite(out32)
+    e = rel_l2(outbf[np.broadcast_to(m, out32.shape)], out32[np.broadcast_to(m, out32.shape)])
+    nd = int((info32["rank"] != infobf["rank"]).sum())
+    print(f"[fullsize] bf16x3 vs fp32: rel-L2 {e:.2e}, rank differences {nd} / {len(info32['rank'])}",
+          flush=True)
+    assert e < 1e-3
+    assert nd <= len(info32["rank"]) // 100
+    sel = infobf["index"].astype(np.int64)
+    lo = (38, 38, 22)
+    hi = tuple(v + 4 for v in lo)
+    ref, _, _ = _window_reference(data, mask, ps, po, lambda X: O.mppca_patch(X, 1.0), sel, lo, hi)
+    win = outbf[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+    ref = np.where(mask[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]][..., None], ref, 0.0)
+    ew = rel_l2(np.nan_to_num(win), np.nan_to_num(ref))
+    print(f"[fullsize] bf16x3 window vs oracle {ew:.2e}", flush=True)
+    assert ew < 1e-3
