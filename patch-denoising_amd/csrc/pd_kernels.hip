@@ -705,6 +705,254 @@ size_t tridiag_lds_bytes(int n) {
          al(4 * 128 * sizeof(E)) + 2 * al(128 * sizeof(R)) + al(128 * sizeof(E));
 }
 
+// ---------------------------------------------------------------------------
+// Gram sizes 129 .. kNBig (e.g. 11^3-voxel patches of 200 frames, or 7^3-voxel
+// patches of 300 frames): the Gram no longer fits one workgroup's registers or
+// LDS, so it lives in the patch's H block in global memory (L2 / MALL
+// resident while the workgroup works on it).
+//   gram_big_kernel     per patch: per-frame means (rows added in order), then
+//                       the Gram in 64 x 64 output tiles (lower tiles, each
+//                       mirrored), reduction staged through LDS 16 at a time
+//   tridiag_big_kernel  per patch: unblocked Householder (LAPACK ?hetd2,
+//                       lower) on the full Hermitian matrix in global memory,
+//                       one thread per column for the product and the rank-2
+//                       update (coalesced row sweeps); reflector k is stored
+//                       in row k, columns k+1 .. n-1, a part of the matrix
+//                       that no later step reads
+// ---------------------------------------------------------------------------
+constexpr int kNBig = 512;  // largest supported Gram size
+constexpr int kGBT = 64;    // Gram output tile
+constexpr int kGBK = 16;    // reduction indices per staged chunk
+
+template <class IN, class E, bool TSIDE>
+__global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vol, GramArgs a) {
+  using R = typename real_of<E>::type;
+  const Geom& g = a.g;
+  const int n = a.n, M = g.M, T = g.pt;
+  const int K = TSIDE ? M : T;  // reduction length
+  const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
+  constexpr int LW = kGBT + 1;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  IN* mean = lds_carve<IN>(cur, T);
+  int64_t* roff = lds_carve<int64_t>(cur, M);
+  E* As = lds_carve<E>(cur, (size_t)kGBK * LW);  // As[k][i]
+  E* Bs = lds_carve<E>(cur, (size_t)kGBK * LW);
+  const int nb = (n + kGBT - 1) / kGBT;
+  const int ntile = nb * (nb + 1) / 2;
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    int64_t sp;
+    int t0;
+    patch_origin(g, a.idx[p], sp, t0);
+    const IN* X = vol + sp + t0;
+    for (int r = tid; r < M; r += kNT) roff[r] = row_offset(g, r);
+    lds_barrier();
+    // per-frame mean over the rows, added in row order (numpy's axis-0 reduction)
+    for (int t = tid; t < T; t += kNT)
+      mean[t] = div_n(seq_add(X[roff[0] + t], X + t, roff + 1, M - 1), M);
+    lds_barrier();
+    E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+    // element (i, k) of the n x K operand, centred: T-side Xc[k][i], M-side Xc[i][k]
+    auto stage = [&](E* S, int b0, int k0, int kc) {
+      for (int q = tid; q < kGBK * kGBT; q += kNT) {
+        int i, k;
+        if (TSIDE) {  // consecutive threads: consecutive frames of one row
+          k = q / kGBT;
+          i = q % kGBT;
+        } else {  // consecutive threads: consecutive frames of one row
+          i = q / kGBK;
+          k = q % kGBK;
+        }
+        E v = E(R(0));
+        if (k < kc && b0 + i < n) {
+          const int rr = TSIDE ? k0 + k : b0 + i, tt = TSIDE ? b0 + i : k0 + k;
+          v = cvt<E>(X[roff[rr] + tt] - mean[tt]);
+        }
+        S[k * LW + i] = v;
+      }
+    };
+    for (int tp = 0; tp < ntile; ++tp) {
+      int bi = 0;
+      while ((bi + 1) * (bi + 2) / 2 <= tp) ++bi;
+      const int bj = tp - bi * (bi + 1) / 2;
+      const int i0 = bi * kGBT, j0 = bj * kGBT;
+      E acc[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = E(R(0));
+      for (int k0 = 0; k0 < K; k0 += kGBK) {
+        const int kc = min(kGBK, K - k0);
+        stage(As, i0, k0, kc);
+        stage(Bs, j0, k0, kc);
+        lds_barrier();
+        for (int k = 0; k < kc; ++k) {
+          E xa[4], xb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xa[u] = As[k * LW + ty + 16 * u];
+            xb[u] = Bs[k * LW + tx + 16 * u];
+            if (TSIDE) xa[u] = conj_(xa[u]);  // G = Xc^H Xc
+            else xb[u] = conj_(xb[u]);        // G = Xc Xc^H
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += xa[u] * xb[v];
+        }
+        lds_barrier();
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = i0 + ty + 16 * u, c = j0 + tx + 16 * v;
+          if (r < n && c < n) {
+            E x = acc[u][v];
+            if (a.scale) x = div_n(x, M - 1);
+            G[(size_t)r * n + c] = x;
+            if (bi != bj) G[(size_t)c * n + r] = conj_(x);
+          }
+        }
+    }
+    lds_barrier();
+  }
+}
+
+template <class IN, class E>
+size_t gram_big_lds_bytes(int M, int T) {
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  return al((size_t)T * sizeof(IN)) + al((size_t)M * sizeof(int64_t)) +
+         2 * al((size_t)kGBK * (kGBT + 1) * sizeof(E));
+}
+
+template <class E>
+__global__ __launch_bounds__(kNT) void tridiag_big_kernel(GramArgs a) {
+  using R = typename real_of<E>::type;
+  const int n = a.n, tid = threadIdx.x;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  E* v = lds_carve<E>(cur, kNBig);
+  E* w = lds_carve<E>(cur, kNBig);
+  R* dd = lds_carve<R>(cur, kNBig);
+  R* ee = lds_carve<R>(cur, kNBig);
+  E* tt = lds_carve<E>(cur, kNBig);
+  R* redr = lds_carve<R>(cur, kNT / kWave);
+  E* rede = lds_carve<E>(cur, kNT / kWave);
+  const E zero = E(R(0));
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    E* A = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+    for (int k = 0; k < n - 1; ++k) {
+      // x = column k below the diagonal = conj(row k) at columns > k
+      E* rowk = A + (size_t)k * n;
+      const E alpha = conj_(rowk[k + 1]);
+      if (tid == 0) dd[k] = re_(rowk[k]);
+      R part = R(0);
+      for (int c = k + 2 + tid; c < n; c += kNT) part += abs2_(rowk[c]);
+      const R xn2 = block_sum<kNT>(part, redr);  // (its barriers order these reads before the stores below)
+      const R ar = re_(alpha), ai = im_(alpha);
+      const bool refl = (xn2 > R(0)) || (ai != R(0));
+      if (!refl) {
+        // H_k = I: nothing to update; reflector row and tau are zero
+        for (int c = k + 1 + tid; c < n; c += kNT) rowk[c] = zero;
+        if (tid == 0) {
+          tt[k] = zero;
+          ee[k] = ar;
+        }
+        __syncthreads();
+        continue;
+      }
+      const R beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
+      const R ibeta = R(1) / beta;
+      const E tk = mk<E>((beta - ar) * ibeta, -ai * ibeta);
+      const E scal = recip_(alpha - E(beta));
+      for (int c = tid; c < n; c += kNT)
+        v[c] = c > k + 1 ? conj_(rowk[c]) * scal : (c == k + 1 ? E(R(1)) : zero);
+      __syncthreads();  // every read of row k precedes the reflector store
+      for (int c = k + 1 + tid; c < n; c += kNT) rowk[c] = v[c];
+      if (tid == 0) {
+        tt[k] = tk;
+        ee[k] = beta;
+      }
+      // p_c = tau sum_{r > k} conj(A[r][c]) v_r, one thread per column
+      E pc[kNBig / kNT];
+#pragma unroll
+      for (int j = 0; j < kNBig / kNT; ++j) {
+        const int c = k + 1 + tid + j * kNT;
+        E sacc = zero;
+        if (c < n) {
+          const E* col = A + c;
+          int r = k + 1;
+          for (; r + 4 <= n; r += 4) {
+            E x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sacc += conj_(x[u]) * v[r + u];
+          }
+          for (; r < n; ++r) sacc += conj_(col[(size_t)r * n]) * v[r];
+        }
+        pc[j] = tk * sacc;
+      }
+      // K = -tau/2 p^H v, w = p + K v
+      E pv = zero;
+#pragma unroll
+      for (int j = 0; j < kNBig / kNT; ++j) {
+        const int c = k + 1 + tid + j * kNT;
+        if (c < n) pv += conj_(pc[j]) * v[c];
+      }
+      pv = block_sum<kNT>(pv, rede);
+      const E kc = (R(-0.5) * tk) * pv;
+#pragma unroll
+      for (int j = 0; j < kNBig / kNT; ++j) {
+        const int c = k + 1 + tid + j * kNT;
+        if (c < n) w[c] = pc[j] + kc * v[c];
+      }
+      __syncthreads();
+      // A -= v w^H + w v^H on the trailing block, one thread per column
+#pragma unroll
+      for (int j = 0; j < kNBig / kNT; ++j) {
+        const int c = k + 1 + tid + j * kNT;
+        if (c < n) {
+          E* col = A + c;
+          const E cw = conj_(w[c]), cv = conj_(v[c]);
+          int r = k + 1;
+          for (; r + 4 <= n; r += 4) {
+            E x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              col[(size_t)(r + u) * n] = rank2_update(x[u], v[r + u], cw, w[r + u], cv);
+          }
+          for (; r < n; ++r) col[(size_t)r * n] = rank2_update(col[(size_t)r * n], v[r], cw, w[r], cv);
+        }
+      }
+      __syncthreads();  // the next step reads row k + 1 written by other threads
+    }
+    if (tid == 0) dd[n - 1] = re_(A[(size_t)(n - 1) * n + n - 1]);
+    __syncthreads();
+    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
+    E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
+    for (int q = tid; q < n; q += kNT) {
+      const R ev = q < n - 1 ? ee[q] : R(0);
+      DEp[q] = dd[q];
+      DEp[n + q] = ev;
+      DEp[2 * n + q] = ev * ev;
+      TAUp[q] = q < n - 1 ? tt[q] : E(R(0));
+    }
+    __syncthreads();
+  }
+}
+
+template <class E>
+size_t tridiag_big_lds_bytes() {
+  using R = typename real_of<E>::type;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  return 3 * al(kNBig * sizeof(E)) + 2 * al(kNBig * sizeof(R)) + al(kNT / kWave * sizeof(R)) +
+         al(kNT / kWave * sizeof(E));
+}
+
 // ===========================================================================
 // 3. Tridiagonal eigen stage + threshold rules (lowrank.py)
 // ===========================================================================
@@ -1058,14 +1306,16 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
   __syncthreads();
 }
 
-constexpr int kKB = 16;  // components per eigenvector batch (registers per lane)
+constexpr int kKB = 16;    // components per eigenvector batch (registers per lane)
+constexpr int kKBBig = 4;  // the same for Grams above 128 (more rows per lane)
 
 // One wavefront per patch.  Eigenvalues by multisection on the tridiagonal,
 // the method's rule, then for the kept components: inverse iteration (one
 // lane per component, LU in scratch), Gram-Schmidt inside eigenvalue
 // clusters, and back-transformation y = H_0 ... H_{n-2} z with the vectors
-// held row-distributed in registers (lane l: rows l and l + 64).
-template <class E, class RT>
+// held row-distributed in registers (lane l: rows l + 64 s, s < NR; batches
+// of up to KBC components).
+template <class E, class RT, int NR = 2, int KBC = kKB>
 __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   using R = typename real_of<E>::type;
   const int lane = threadIdx.x;
@@ -1300,7 +1550,8 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
     // 1e-8 tnorm (error <= ~1e-8); FP32 keeps the classical 1e-3 tnorm.
     const R clus = (sizeof(R) == 8 ? R(1e-8) : R(1e-3)) * tnorm;
     const int n_init = 3;  // initial solves per vector
-    const int i0 = lane, i1 = lane + kWave;  // rows held by this lane
+    // rows held by this lane: lane + 64 s, s < NR
+    auto row_of = [&](int sl) { return lane + kWave * sl; };
     for (int c0 = 0; c0 < ncomp; c0 += KB) {
       const int kb = min(KB, ncomp - c0);
       // inverse iteration, one lane per component
@@ -1321,18 +1572,28 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         for (int it = 0; it < n_init; ++it) solve(lane);
       }
       wave_sync();
-      // rows -> registers (lane l: rows l and l+64)
-      R z0[kKB], z1[kKB];
+      // rows -> registers
+      R z[NR][KBC];
+      auto load_z = [&]() {
 #pragma unroll
-      for (int c = 0; c < kKB; ++c) {
-        z0[c] = (c < kb && i0 < n) ? zb[i0 * KB + c] : R(0);
-        z1[c] = (c < kb && i1 < n) ? zb[i1 * KB + c] : R(0);
-      }
+        for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+          for (int c = 0; c < KBC; ++c)
+            z[sl][c] = (c < kb && row_of(sl) < n) ? zb[row_of(sl) * KB + c] : R(0);
+      };
+      load_z();
+      // sum over this lane's rows of x_s y_s, then over the wavefront
+      auto dot_rows = [&](auto fx, auto fy) {
+        R t = fx(0) * fy(0);
+#pragma unroll
+        for (int sl = 1; sl < NR; ++sl) t += fx(sl) * fy(sl);
+        return dpp_sum(t);
+      };
       // classical Gram-Schmidt, twice, inside eigenvalue clusters; one extra
       // inverse-iteration step in between
       for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
-        for (int c = 0; c < kKB; ++c) {
+        for (int c = 0; c < KBC; ++c) {
           if (c >= kb) continue;
           const int gc = c0 + c;
           // earlier batches (stored z in global memory).  The components are
@@ -1343,80 +1604,84 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           while (qs > 0 && fabs(lam[qs - 1] - lam[gc]) <= clus) --qs;
           for (int q = qs; q < c0; ++q) {
             if (fabs(lam[q] - lam[gc]) > clus) continue;
-            const R zq0 = i0 < n ? Zp[(size_t)i0 * a.kmax + q] : R(0);
-            const R zq1 = i1 < n ? Zp[(size_t)i1 * a.kmax + q] : R(0);
-            const R dot = dpp_sum(zq0 * z0[c] + zq1 * z1[c]);
-            z0[c] -= dot * zq0;
-            z1[c] -= dot * zq1;
+            R zq[NR];
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl)
+              zq[sl] = row_of(sl) < n ? Zp[(size_t)row_of(sl) * a.kmax + q] : R(0);
+            const R dot = dot_rows([&](int sl) { return zq[sl]; }, [&](int sl) { return z[sl][c]; });
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) z[sl][c] -= dot * zq[sl];
           }
 #pragma unroll
-          for (int q = 0; q < kKB; ++q) {
+          for (int q = 0; q < KBC; ++q) {
             if (q >= c) break;
             if (fabs(lam[c0 + q] - lam[gc]) > clus) continue;
-            const R dot = dpp_sum(z0[q] * z0[c] + z1[q] * z1[c]);
-            z0[c] -= dot * z0[q];
-            z1[c] -= dot * z1[q];
+            const R dot = dot_rows([&](int sl) { return z[sl][q]; }, [&](int sl) { return z[sl][c]; });
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) z[sl][c] -= dot * z[sl][q];
           }
-          const R inv = R(1) / sqrt(fmax(dpp_sum(z0[c] * z0[c] + z1[c] * z1[c]), Lim<R>::tiny()));
-          z0[c] *= inv;
-          z1[c] *= inv;
+          const R nrm = dot_rows([&](int sl) { return z[sl][c]; }, [&](int sl) { return z[sl][c]; });
+          const R inv = R(1) / sqrt(fmax(nrm, Lim<R>::tiny()));
+#pragma unroll
+          for (int sl = 0; sl < NR; ++sl) z[sl][c] *= inv;
         }
         if (pass == 0) {
 #pragma unroll
-          for (int c = 0; c < kKB; ++c) {
-            if (c < kb && i0 < n) zb[i0 * KB + c] = z0[c];
-            if (c < kb && i1 < n) zb[i1 * KB + c] = z1[c];
-          }
+          for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+            for (int c = 0; c < KBC; ++c)
+              if (c < kb && row_of(sl) < n) zb[row_of(sl) * KB + c] = z[sl][c];
           wave_sync();
           if (lane < kb) solve(lane);
           wave_sync();
-#pragma unroll
-          for (int c = 0; c < kKB; ++c) {
-            z0[c] = (c < kb && i0 < n) ? zb[i0 * KB + c] : R(0);
-            z1[c] = (c < kb && i1 < n) ? zb[i1 * KB + c] : R(0);
-          }
+          load_z();
         }
       }
       // keep z for clustering against later batches (read back by other lanes)
       if (c0 + kb < ncomp) {
 #pragma unroll
-        for (int c = 0; c < kKB; ++c) {
-          if (c < kb && i0 < n) Zp[(size_t)i0 * a.kmax + c0 + c] = z0[c];
-          if (c < kb && i1 < n) Zp[(size_t)i1 * a.kmax + c0 + c] = z1[c];
-        }
+        for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+          for (int c = 0; c < KBC; ++c)
+            if (c < kb && row_of(sl) < n) Zp[(size_t)row_of(sl) * a.kmax + c0 + c] = z[sl][c];
         __syncthreads();
       }
-      // back-transformation y = H_0 ... H_{n-2} z, reflectors prefetched 8 ahead
-      E y0[kKB], y1[kKB];
+      // back-transformation y = H_0 ... H_{n-2} z, reflectors prefetched PF ahead
+      E y[NR][KBC];
 #pragma unroll
-      for (int c = 0; c < kKB; ++c) {
-        y0[c] = E(z0[c]);
-        y1[c] = E(z1[c]);
-      }
-      constexpr int PF = 8;
-      E va[PF], vb[PF], tv[PF];
+      for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+        for (int c = 0; c < KBC; ++c) y[sl][c] = E(z[sl][c]);
+      constexpr int PF = NR > 2 ? 2 : 8;
+      E vr[NR][PF], tv[PF];
       auto fetch = [&](int kbase) {  // reflectors kbase, kbase-1, ..., kbase-PF+1
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int k = kbase - u;
           if (k >= 0) {
-            const E* row = Hp + hh_row(k, n) - (k + 1);  // indexed by absolute column
+            // reflector k indexed by absolute column: packed rows (n <= 128) or
+            // row k of the full matrix (larger Grams)
+            const E* row = NR > 2 ? Hp + (size_t)k * n : Hp + hh_row(k, n) - (k + 1);
             tv[u] = TAUp[k];
-            va[u] = (i0 > k + 1 && i0 < n) ? row[i0] : E(R(i0 == k + 1 ? 1 : 0));
-            vb[u] = (i1 > k + 1 && i1 < n) ? row[i1] : E(R(i1 == k + 1 ? 1 : 0));
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) {
+              const int i = row_of(sl);
+              vr[sl][u] = (i > k + 1 && i < n) ? row[i] : E(R(i == k + 1 ? 1 : 0));
+            }
           } else {
             tv[u] = E(R(0));
-            va[u] = vb[u] = E(R(0));
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) vr[sl][u] = E(R(0));
           }
         }
       };
       fetch(n - 2);
       for (int kbase = n - 2; kbase >= 0; kbase -= PF) {
-        E ca[PF], cb[PF], ct[PF];
+        E cv[NR][PF], ct[PF];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-          ca[u] = va[u];
-          cb[u] = vb[u];
+#pragma unroll
+          for (int sl = 0; sl < NR; ++sl) cv[sl][u] = vr[sl][u];
           ct[u] = tv[u];
         }
         if (kbase - PF >= 0) fetch(kbase - PF);
@@ -1425,19 +1690,22 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           const E tk = ct[u];
           if (re_(tk) == R(0) && im_(tk) == R(0)) continue;
 #pragma unroll
-          for (int c = 0; c < kKB; ++c) {
+          for (int c = 0; c < KBC; ++c) {
             if (c >= kb) break;
-            const E sdot = tk * dpp_sum(conj_(ca[u]) * y0[c] + conj_(cb[u]) * y1[c]);
-            y0[c] -= ca[u] * sdot;
-            y1[c] -= cb[u] * sdot;
+            E t = conj_(cv[0][u]) * y[0][c];
+#pragma unroll
+            for (int sl = 1; sl < NR; ++sl) t += conj_(cv[sl][u]) * y[sl][c];
+            const E sdot = tk * dpp_sum(t);
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) y[sl][c] -= cv[sl][u] * sdot;
           }
         }
       }
 #pragma unroll
-      for (int c = 0; c < kKB; ++c) {
-        if (c < kb && i0 < n) Yp[(size_t)i0 * a.kmax + c0 + c] = y0[c];
-        if (c < kb && i1 < n) Yp[(size_t)i1 * a.kmax + c0 + c] = y1[c];
-      }
+      for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+        for (int c = 0; c < KBC; ++c)
+          if (c < kb && row_of(sl) < n) Yp[(size_t)row_of(sl) * a.kmax + c0 + c] = y[sl][c];
       wave_sync();
     }
     // ---------------- per-patch scalars
@@ -1550,10 +1818,11 @@ constexpr int kSynSC = 32;  // frames (M-side) or rows (T-side) per staged chunk
 constexpr int kSynKG = 16;  // components per group held in LDS
 constexpr int kSynNI = 16;  // output items per thread per chunk (128 x 32 / 256)
 
-template <class IN, class E, bool TSIDE>
+// SC frames (M-side) or rows (T-side) per staged chunk and KG components per
+// group: SC * n <= kSynNI * kNT (one staged chunk = kSynNI items per thread).
+template <class IN, class E, bool TSIDE, int SC = kSynSC, int KG = kSynKG>
 __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, SynthArgs a) {
   using R = typename real_of<E>::type;
-  constexpr int SC = kSynSC, KG = kSynKG;
   const Geom& g = a.g;
   const int n = a.n, tid = threadIdx.x;
   const int M = g.M, T = g.pt;
@@ -1590,7 +1859,7 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
         const int tc = min(SC, T - c0);
         if (center && (ctr_t < c0 || ctr_t >= c0 + tc)) continue;
         {
-          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          constexpr int NL = kSynNI;
           IN buf[NL];
 #pragma unroll
           for (int u = 0; u < NL; ++u) {
@@ -1663,14 +1932,18 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
         lds_barrier();
       }
     } else {
-      // pass 1: per-frame means over all rows (sequential order)
-      IN sm = IN(0);
+      // pass 1: per-frame means over all rows (sequential order); thread tid
+      // sums frames tid + kNT j
+      constexpr int NF = (SC == kSynSC) ? 1 : kNBig / kNT;  // frames per thread (T <= NF kNT)
+      IN sm[NF];
+#pragma unroll
+      for (int j = 0; j < NF; ++j) sm[j] = IN(0);
       for (int r0 = 0; r0 < M; r0 += SC) {
         const int rc = min(SC, M - r0);
         for (int r = tid; r < rc; r += kNT) roff[r] = row_offset(g, r0 + r);
         lds_barrier();
         {
-          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          constexpr int NL = kSynNI;
           IN buf[NL];
 #pragma unroll
           for (int u = 0; u < NL; ++u) {
@@ -1684,13 +1957,19 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
           }
         }
         lds_barrier();
-        if (tid < T) {
-          if (r0 == 0) sm = seq_add(Sraw[tid], Sraw + ldw + tid, rc - 1, ldw);
-          else sm = seq_add(sm, Sraw + tid, rc, ldw);
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+          const int t = tid + j * kNT;
+          if (t < T) {
+            if (r0 == 0) sm[j] = seq_add(Sraw[t], Sraw + ldw + t, rc - 1, ldw);
+            else sm[j] = seq_add(sm[j], Sraw + t, rc, ldw);
+          }
         }
         lds_barrier();
       }
-      for (int t = tid; t < T; t += kNT) mean[t] = div_n(sm, M);
+#pragma unroll
+      for (int j = 0; j < NF; ++j)
+        if (tid + j * kNT < T) mean[tid + j * kNT] = div_n(sm[j], M);
       lds_barrier();
       for (int r0 = 0; r0 < M; r0 += SC) {
         const int rc = min(SC, M - r0);
@@ -1698,7 +1977,7 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
         for (int r = tid; r < rc; r += kNT) roff[r] = row_offset(g, r0 + r);
         lds_barrier();
         {
-          constexpr int NL = (SC * 128 + kNT - 1) / kNT;
+          constexpr int NL = kSynNI;
           IN buf[NL];
 #pragma unroll
           for (int u = 0; u < NL; ++u) {
@@ -1776,10 +2055,9 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
   }
 }
 
-template <class IN, class E, bool TSIDE>
+template <class IN, class E, bool TSIDE, int SC = kSynSC, int KG = kSynKG>
 size_t synth_lds_bytes(int n, int kmax, int M, int T) {
   using R = typename real_of<E>::type;
-  constexpr int SC = kSynSC, KG = kSynKG;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
   const int ldw = TSIDE ? (T + 1) : (M + 1);
   size_t s = al((size_t)n * KG * sizeof(E)) + al(KG * sizeof(R));
@@ -1944,13 +2222,13 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   pl->tside = pt < M;
   pl->n = pl->tside ? pt : M;
   const int n = pl->n;
-  if (n > 128)
+  if (n > kNBig)
     return fail(PDAMD_EUNSUPPORTED,
-                "min(patch voxels, patch frames) = %d exceeds 128 (supported maximum)", n);
+                "min(patch voxels, patch frames) = %d exceeds %d (supported maximum)", n, kNBig);
   if (n < 2) return fail(PDAMD_EUNSUPPORTED, "patch too small (min side %d)", n);
   if ((P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) && pt < 10)
     return fail(PDAMD_EINVAL, "the eigen rules need at least 10 frames per patch");
-  pl->tile = n <= 32 ? 2 : (n <= 64 ? 4 : 8);
+  pl->tile = n <= 32 ? 2 : (n <= 64 ? 4 : (n <= 128 ? 8 : 0));  // 0: large-Gram kernels
   const bool cplx = P->dtype == PDAMD_C64 || P->dtype == PDAMD_C128;
   const bool in64 = P->dtype == PDAMD_F64 || P->dtype == PDAMD_C128;
   pl->rule_f64 = in64 ? 1 : 0;
@@ -1976,9 +2254,11 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
     if (!pl->tside)
       return fail(PDAMD_EUNSUPPORTED,
                   "the backward pass needs fewer frames than patch voxels (T-side Gram)");
+    if (n > 128)
+      return fail(PDAMD_EUNSUPPORTED, "the backward pass supports min(patch voxels, frames) <= 128");
     pl->kmax = n;
   }
-  pl->kb = std::min(kKB, pl->kmax);
+  pl->kb = std::min(pl->tile ? kKB : kKBBig, pl->kmax);
   // chunk: keep the per-chunk buffers under ~4 GiB (6 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
@@ -2115,8 +2395,31 @@ int launch_tridiag(const Plan& pl, GramArgs ga, hipStream_t st) {
 }
 
 template <class IN, class E, bool TSIDE>
+int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
+  const size_t lds = gram_big_lds_bytes<IN, E>(pl.g.M, pl.g.pt);
+  if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
+  auto kern = gram_big_kernel<IN, E, TSIDE>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max(1, std::min(4, (int)((160 * 1024) / lds)));
+  kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class E>
+int launch_tridiag_big(const Plan& pl, GramArgs ga, hipStream_t st) {
+  const size_t lds = tridiag_big_lds_bytes<E>();
+  auto kern = tridiag_big_kernel<E>;
+  HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  kern<<<grid_for(ga.np, 2), kNT, lds, st>>>(ga);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+template <class IN, class E, bool TSIDE>
 int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
   switch (pl.tile) {
+    case 0: return launch_gram_big<IN, E, TSIDE>(pl, vol, ga, st);
     case 2: return launch_gram<IN, E, TSIDE, 2>(pl, vol, ga, st);
     case 4: return launch_gram<IN, E, TSIDE, 4>(pl, vol, ga, st);
     default:
@@ -2129,6 +2432,7 @@ int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t s
 template <class E>
 int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
   switch (pl.tile) {
+    case 0: return launch_tridiag_big<E>(pl, ga, st);
     case 2: return launch_tridiag<E, 8>(pl, ga, st);
     case 4: return launch_tridiag<E, 16>(pl, ga, st);
     default: return launch_tridiag<E, 32>(pl, ga, st);
@@ -2139,7 +2443,9 @@ template <class E, class RT>
 int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
   const size_t lds = trieig_lds_bytes<E>(pl.n, pl.kb);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "eigen LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = trieig_kernel<E, RT>;
+  auto kern = pl.tile ? trieig_kernel<E, RT>
+                      : (pl.n <= 4 * kWave ? trieig_kernel<E, RT, 4, kKBBig>
+                                           : trieig_kernel<E, RT, kNBig / kWave, kKBBig>);
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(16, (int)((160 * 1024) / lds)));
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
@@ -2156,11 +2462,17 @@ int launch_eig_all(const Plan& pl, EigArgs ea, hipStream_t st) {
   return PDAMD_OK;
 }
 
+constexpr int kSynSCBig = 8, kSynKGBig = 4;  // synth chunking for Grams above 128
+static_assert(kSynSC * 128 <= kSynNI * kNT && kSynSCBig * kNBig <= kSynNI * kNT,
+              "one staged synthesis chunk must fit kSynNI items per thread");
+
 template <class IN, class E, bool TSIDE>
 int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) {
-  const size_t lds = synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
+  const bool big = pl.tile == 0;
+  const size_t lds = big ? synth_lds_bytes<IN, E, TSIDE, kSynSCBig, kSynKGBig>(pl.n, pl.kmax, pl.g.M, pl.g.pt)
+                         : synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "synth LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = synth_kernel<IN, E, TSIDE>;
+  auto kern = big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
   kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);

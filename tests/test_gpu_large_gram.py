@@ -1,0 +1,100 @@
+"""GPU parity for Gram sizes n = min(patch voxels, frames) above 128.
+
+Patch shapes of the full-size workloads whose Gram exceeds 128 -- 11^3 x 200
+complex NORDIC patches, 7^3-voxel patches of 200 to 400 frames, 6^3 x 300,
+15^2 x 300 in 2-D -- on volumes just a few patches wide, so that the oracle
+(a NumPy restatement of the reference, SVD / eigh per patch) finishes in
+seconds.  Both Gram sides are covered: T-side (frames < voxels, G = Xc^H Xc)
+and M-side (voxels <= frames, G = Xc Xc^H).
+
+Tolerances as for the small Grams: relative L2 over finite entries with the
+NaN pattern required to match; 1e-5 for float32 / complex64 data, 5e-7 for
+float64 (the float32 weight divisor differs by float32 ulps with the
+summation order); weights 1e-6.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(got, want):
+    fin = np.isfinite(want)
+    d = np.asarray(got, np.complex128)[fin] - np.asarray(want, np.complex128)[fin]
+    den = np.linalg.norm(np.asarray(want, np.complex128)[fin])
+    return float(np.linalg.norm(d) / max(den, 1e-300))
+
+
+def _lowrank(shape, T, rank, seed, dtype=np.float32, sigma=30.0, base=800.0, amp=60.0):
+    rng = np.random.default_rng(seed)
+    maps = rng.standard_normal((rank,) + shape)
+    tcs = rng.standard_normal((rank, T))
+    clean = base + amp * np.tensordot(maps, tcs, axes=([0], [0]))
+    return (clean + sigma * rng.standard_normal(shape + (T,))).astype(dtype)
+
+
+def _sigma_map(shape, s0=30.0):
+    grids = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+    rad = np.sqrt(sum((g - (s - 1) / 2.0) ** 2 for g, s in zip(grids, shape)))
+    return s0 * (1.0 + 0.5 * rad / max(rad.max(), 1e-9))
+
+
+def _cases():
+    from patch_denoise import workloads
+
+    c = []
+    # T-side, n = 200 (7^3 = 343 voxels > 200 frames), MP-PCA float32
+    c.append(("mppca_f32_p7_T200", "mp_pca", _lowrank((9, 9, 8), 200, 6, 1), (7, 6), {}))
+    # M-side, n = 216 (6^3 voxels < 260 frames), MP-PCA float64
+    c.append(("mppca_f64_p6_T260", "mp_pca",
+              _lowrank((8, 8, 7), 260, 5, 2, dtype=np.float64), (6, 5), {}))
+    # M-side, n = 343, Hybrid-PCA with a per-voxel sigma map
+    d = _lowrank((8, 8, 8), 400, 4, 3)
+    c.append(("hybrid_f32_p7_T400", "hybrid_pca", d, (7, 6),
+              {"noise_std": _sigma_map(d.shape[:-1])}))
+    # M-side, n = 216, raw SVT, spatial mask, average recombination
+    d = _lowrank((8, 8, 7), 300, 5, 4)
+    mask = np.ones(d.shape[:-1], bool)
+    mask[0, :, :] = False
+    c.append(("raw_f32_p6_T300_mask_avg", "raw_svt", d, (6, 5),
+              {"threshold": 30.0 * (np.sqrt(216) + np.sqrt(300)) * 1.05, "mask": mask,
+               "recombination": "average"}))
+    # T-side, n = 300 (beta = 300 / 343), optimal shrinkage, Frobenius and nuclear
+    d = _lowrank((8, 8, 8), 300, 6, 5)
+    c.append(("optimal_fro_f32_p7_T300", "optimal_thresholding", d, (7, 6), {"loss": "fro"}))
+    c.append(("optimal_nuc_f32_p7_T300_center", "optimal_thresholding", d, (7, 6),
+              {"loss": "nuc", "recombination": "center"}))
+    # 2-D: 15^2 = 225 voxels < 300 frames -> M-side n = 225, MP-PCA
+    c.append(("mppca_2d_f32_p15_T300", "mp_pca", _lowrank((17, 17), 300, 4, 6), (15, 14), {}))
+    # complex64 NORDIC on 11^3 x 200 patches (T-side, n = 200)
+    d = workloads.config3(shape=(12, 12, 11), T=200)
+    c.append(("nordic_c64_p11_T200", "nordic", d, (11, 10),
+              {"noise_std": 25.0, "n_iter_threshold": 1}))
+    return c
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_large_gram_matches_oracle(case):
+    import patch_denoise as pd
+    from oracle import cpu_ref as O
+
+    name, fn, data, (ps, po), kw = case
+    n = min(ps ** (data.ndim - 1), data.shape[-1])
+    assert n > 128
+    print(f"[large-gram] {name}: shape {data.shape} {data.dtype} patch {ps} overlap {po} n {n}",
+          flush=True)
+    np.random.seed(0)
+    out, w, _, _ = getattr(pd, fn)(data, ps, po, **kw)
+    np.random.seed(0)
+    want, want_w, _, _ = getattr(O, fn)(data, ps, po, **kw)
+    assert out.dtype == want.dtype and out.shape == want.shape
+    assert np.array_equal(np.isnan(out), np.isnan(want)), "NaN pattern"
+    e = rel_l2(out, want)
+    ew = rel_l2(w, want_w)
+    tol = 5e-7 if data.dtype in (np.float64, np.complex128) else 1e-5
+    print(f"[large-gram]   rel-L2 out {e:.3g} weights {ew:.3g} (tolerance {tol:g})", flush=True)
+    assert e <= tol and ew <= 1e-6
