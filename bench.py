@@ -181,8 +181,8 @@ def bench_differentiable(steps):
 
 
 def bench_batched(steps):
-    """The eight-volume mixed workload (seed 4; volumes whose Gram size
-    exceeds 128 are listed as skipped): one pdamd_denoise call per volume vs
+    """The eight-volume mixed workload (seed 4, all eight volumes, Gram sizes
+    27 to 200): one pdamd_denoise call per volume vs
     one pdamd_denoise_batch call (eager, and replayed from its captured
     graph), inputs resident in HBM."""
     import torch
@@ -260,7 +260,7 @@ def main():
                     help="skip the plain one-patch-per-launch GPU baseline")
     ap.add_argument("--plain-stride", type=int, default=64)
     ap.add_argument("--precision", default="auto")
-    ap.add_argument("--batch-steps", type=int, default=3,
+    ap.add_argument("--batch-steps", type=int, default=2,
                     help="steps of the batched multi-volume measurement; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")

@@ -826,39 +826,65 @@ size_t gram_big_lds_bytes(int M, int T) {
          2 * al((size_t)kGBK * (kGBT + 1) * sizeof(E));
 }
 
+// Large-Gram Householder: steps k < k0 on the full matrix in global memory,
+// then (once the trailing block A[k0:, k0:] fits) the rest on that block held
+// in LDS as a packed lower triangle (column j of the m0 = n - k0 block holds
+// rows j .. m0-1 from pk_col(j)).  Both phases compute, per step, the
+// reflector from column k, p = tau A v, w = p - tau/2 (p^H v) v and
+// A -= v w^H + w v^H on the trailing block; d, e, e^2 and tau go straight to
+// DE / TAU and v_k to row k of the global block (columns k+1 .. n-1).
+constexpr int kTBT = 512;                 // threads of the large-Gram tridiagonalisation
+constexpr size_t kTBLds = 160 * 1024;     // its LDS
+__host__ __device__ __forceinline__ int pk_col(int j, int m) { return j * m - (j * (j - 1)) / 2; }
+
+// Smallest k0 with the packed (n - k0)-block, v and w within the LDS.
+inline int tridiag_big_k0(int n, size_t esz) {
+  for (int k0 = 0; k0 < n; ++k0) {
+    const size_t m = (size_t)(n - k0);
+    if ((m * (m + 1) / 2 + 2 * m + 8) * esz + 256 <= kTBLds) return k0;
+  }
+  return n;
+}
+
 template <class E>
-__global__ __launch_bounds__(kNT) void tridiag_big_kernel(GramArgs a) {
+__global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
   using R = typename real_of<E>::type;
+  constexpr int NT = kTBT;
   const int n = a.n, tid = threadIdx.x;
+  const int m0 = n - k0;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  E* v = lds_carve<E>(cur, kNBig);
-  E* w = lds_carve<E>(cur, kNBig);
-  R* dd = lds_carve<R>(cur, kNBig);
-  R* ee = lds_carve<R>(cur, kNBig);
-  E* tt = lds_carve<E>(cur, kNBig);
-  R* redr = lds_carve<R>(cur, kNT / kWave);
-  E* rede = lds_carve<E>(cur, kNT / kWave);
+  __shared__ R redr[NT / kWave];
+  __shared__ E rede[NT / kWave];
+  E* Bp = lds_carve<E>(cur, (size_t)m0 * (m0 + 1) / 2);  // packed trailing block
+  E* v = lds_carve<E>(cur, m0);
+  E* w = lds_carve<E>(cur, m0);
+  // the global phase keeps v and w (length n) in the still unused packed area
+  E* vg = Bp;
+  E* wg = Bp + n;
   const E zero = E(R(0));
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     E* A = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
-    for (int k = 0; k < n - 1; ++k) {
-      // x = column k below the diagonal = conj(row k) at columns > k
-      E* rowk = A + (size_t)k * n;
+    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
+    E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
+    auto put_step = [&](int k, R d, R e, E t) {
+      DEp[k] = d;
+      DEp[n + k] = e;
+      DEp[2 * n + k] = e * e;
+      TAUp[k] = t;
+    };
+    // ---------------- global phase, steps k < k0
+    for (int k = 0; k < k0 && k < n - 1; ++k) {
+      E* rowk = A + (size_t)k * n;  // conj(row k) = column k (the matrix is kept Hermitian)
       const E alpha = conj_(rowk[k + 1]);
-      if (tid == 0) dd[k] = re_(rowk[k]);
+      const R dk = re_(rowk[k]);
       R part = R(0);
-      for (int c = k + 2 + tid; c < n; c += kNT) part += abs2_(rowk[c]);
-      const R xn2 = block_sum<kNT>(part, redr);  // (its barriers order these reads before the stores below)
+      for (int c = k + 2 + tid; c < n; c += NT) part += abs2_(rowk[c]);
+      const R xn2 = block_sum<NT>(part, redr);  // its barriers order the reads above before the stores below
       const R ar = re_(alpha), ai = im_(alpha);
-      const bool refl = (xn2 > R(0)) || (ai != R(0));
-      if (!refl) {
-        // H_k = I: nothing to update; reflector row and tau are zero
-        for (int c = k + 1 + tid; c < n; c += kNT) rowk[c] = zero;
-        if (tid == 0) {
-          tt[k] = zero;
-          ee[k] = ar;
-        }
+      if (!((xn2 > R(0)) || (ai != R(0)))) {  // H_k = I
+        for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = zero;
+        if (tid == 0) put_step(k, dk, ar, zero);
         __syncthreads();
         continue;
       }
@@ -866,19 +892,15 @@ __global__ __launch_bounds__(kNT) void tridiag_big_kernel(GramArgs a) {
       const R ibeta = R(1) / beta;
       const E tk = mk<E>((beta - ar) * ibeta, -ai * ibeta);
       const E scal = recip_(alpha - E(beta));
-      for (int c = tid; c < n; c += kNT)
-        v[c] = c > k + 1 ? conj_(rowk[c]) * scal : (c == k + 1 ? E(R(1)) : zero);
-      __syncthreads();  // every read of row k precedes the reflector store
-      for (int c = k + 1 + tid; c < n; c += kNT) rowk[c] = v[c];
-      if (tid == 0) {
-        tt[k] = tk;
-        ee[k] = beta;
-      }
-      // p_c = tau sum_{r > k} conj(A[r][c]) v_r, one thread per column
-      E pc[kNBig / kNT];
+      for (int c = tid; c < n; c += NT)
+        vg[c] = c > k + 1 ? conj_(rowk[c]) * scal : (c == k + 1 ? E(R(1)) : zero);
+      __syncthreads();
+      for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = vg[c];
+      if (tid == 0) put_step(k, dk, beta, tk);
+      E pc[kNBig / NT];
 #pragma unroll
-      for (int j = 0; j < kNBig / kNT; ++j) {
-        const int c = k + 1 + tid + j * kNT;
+      for (int j = 0; j < kNBig / NT; ++j) {
+        const int c = k + 1 + tid + j * NT;
         E sacc = zero;
         if (c < n) {
           const E* col = A + c;
@@ -888,34 +910,32 @@ __global__ __launch_bounds__(kNT) void tridiag_big_kernel(GramArgs a) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) sacc += conj_(x[u]) * v[r + u];
+            for (int u = 0; u < 4; ++u) sacc += conj_(x[u]) * vg[r + u];
           }
-          for (; r < n; ++r) sacc += conj_(col[(size_t)r * n]) * v[r];
+          for (; r < n; ++r) sacc += conj_(col[(size_t)r * n]) * vg[r];
         }
         pc[j] = tk * sacc;
       }
-      // K = -tau/2 p^H v, w = p + K v
       E pv = zero;
 #pragma unroll
-      for (int j = 0; j < kNBig / kNT; ++j) {
-        const int c = k + 1 + tid + j * kNT;
-        if (c < n) pv += conj_(pc[j]) * v[c];
+      for (int j = 0; j < kNBig / NT; ++j) {
+        const int c = k + 1 + tid + j * NT;
+        if (c < n) pv += conj_(pc[j]) * vg[c];
       }
-      pv = block_sum<kNT>(pv, rede);
+      pv = block_sum<NT>(pv, rede);
       const E kc = (R(-0.5) * tk) * pv;
 #pragma unroll
-      for (int j = 0; j < kNBig / kNT; ++j) {
-        const int c = k + 1 + tid + j * kNT;
-        if (c < n) w[c] = pc[j] + kc * v[c];
+      for (int j = 0; j < kNBig / NT; ++j) {
+        const int c = k + 1 + tid + j * NT;
+        if (c < n) wg[c] = pc[j] + kc * vg[c];
       }
       __syncthreads();
-      // A -= v w^H + w v^H on the trailing block, one thread per column
 #pragma unroll
-      for (int j = 0; j < kNBig / kNT; ++j) {
-        const int c = k + 1 + tid + j * kNT;
+      for (int j = 0; j < kNBig / NT; ++j) {
+        const int c = k + 1 + tid + j * NT;
         if (c < n) {
           E* col = A + c;
-          const E cw = conj_(w[c]), cv = conj_(v[c]);
+          const E cw = conj_(wg[c]), cv = conj_(vg[c]);
           int r = k + 1;
           for (; r + 4 <= n; r += 4) {
             E x[4];
@@ -923,34 +943,93 @@ __global__ __launch_bounds__(kNT) void tridiag_big_kernel(GramArgs a) {
             for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-              col[(size_t)(r + u) * n] = rank2_update(x[u], v[r + u], cw, w[r + u], cv);
+              col[(size_t)(r + u) * n] = rank2_update(x[u], vg[r + u], cw, wg[r + u], cv);
           }
-          for (; r < n; ++r) col[(size_t)r * n] = rank2_update(col[(size_t)r * n], v[r], cw, w[r], cv);
+          for (; r < n; ++r) col[(size_t)r * n] = rank2_update(col[(size_t)r * n], vg[r], cw, wg[r], cv);
         }
       }
-      __syncthreads();  // the next step reads row k + 1 written by other threads
+      __syncthreads();
     }
-    if (tid == 0) dd[n - 1] = re_(A[(size_t)(n - 1) * n + n - 1]);
+    // ---------------- LDS phase: load the lower triangle of A[k0:, k0:]
+    for (int i = 0; i < m0; ++i) {
+      const E* row = A + (size_t)(k0 + i) * n + k0;
+      for (int j = tid; j <= i; j += NT) Bp[pk_col(j, m0) + (i - j)] = row[j];
+    }
     __syncthreads();
-    R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
-    E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
-    for (int q = tid; q < n; q += kNT) {
-      const R ev = q < n - 1 ? ee[q] : R(0);
-      DEp[q] = dd[q];
-      DEp[n + q] = ev;
-      DEp[2 * n + q] = ev * ev;
-      TAUp[q] = q < n - 1 ? tt[q] : E(R(0));
+    for (int kk = 0; kk < m0 - 1; ++kk) {
+      const int k = k0 + kk;
+      const E* colk = Bp + pk_col(kk, m0) - kk;  // colk[i] = B[i][kk], i >= kk
+      const E alpha = colk[kk + 1];
+      const R dk = re_(colk[kk]);
+      R part = R(0);
+      for (int i = kk + 2 + tid; i < m0; i += NT) part += abs2_(colk[i]);
+      const R xn2 = block_sum<NT>(part, redr);
+      const R ar = re_(alpha), ai = im_(alpha);
+      E* rowk = A + (size_t)k * n + k0;  // global row k, local columns
+      if (!((xn2 > R(0)) || (ai != R(0)))) {
+        for (int i = kk + 1 + tid; i < m0; i += NT) rowk[i] = zero;
+        if (tid == 0) put_step(k, dk, ar, zero);
+        __syncthreads();
+        continue;
+      }
+      const R beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
+      const R ibeta = R(1) / beta;
+      const E tk = mk<E>((beta - ar) * ibeta, -ai * ibeta);
+      const E scal = recip_(alpha - E(beta));
+      for (int i = tid; i < m0; i += NT) {
+        const E vi = i > kk + 1 ? colk[i] * scal : (i == kk + 1 ? E(R(1)) : zero);
+        v[i] = vi;
+        if (i > kk) rowk[i] = vi;
+      }
+      if (tid == 0) put_step(k, dk, beta, tk);
+      __syncthreads();
+      // p_c = tau (sum_{kk < r < c} B[c][r] v_r + sum_{r >= c} conj(B[r][c]) v_r)
+      E pcv = zero;
+      const int c = kk + 1 + tid;
+      if (c < m0) {
+        E sacc = zero;
+        int idx = pk_col(kk + 1, m0) + (c - kk - 1);  // B[c][kk+1]
+        for (int r = kk + 1; r < c; ++r) {
+          sacc += Bp[idx] * v[r];
+          idx += m0 - r - 1;  // B[c][r] -> B[c][r+1]
+        }
+        const E* colc = Bp + pk_col(c, m0) - c;
+        for (int r = c; r < m0; ++r) sacc += conj_(colc[r]) * v[r];
+        pcv = tk * sacc;
+      }
+      E pv = block_sum<NT>(c < m0 ? conj_(pcv) * v[c < m0 ? c : 0] : zero, rede);
+      const E kc = (R(-0.5) * tk) * pv;
+      if (c < m0) w[c] = pcv + kc * v[c];
+      __syncthreads();
+      // B -= v w^H + w v^H on the lower triangle of the trailing block; thread t
+      // takes columns kk+1+t and m0-1-t (equal total length)
+      const int L = m0 - kk - 1;
+      if (tid < (L + 1) / 2) {
+        const int ca = kk + 1 + tid, cb = m0 - 1 - tid;
+        for (int h = 0; h < 2; ++h) {
+          const int cc = h ? cb : ca;
+          if (h && cb == ca) break;
+          E* colc = Bp + pk_col(cc, m0) - cc;
+          const E cw = conj_(w[cc]), cv = conj_(v[cc]);
+          for (int r = cc; r < m0; ++r) colc[r] = rank2_update(colc[r], v[r], cw, w[r], cv);
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      put_step(n - 1, re_(Bp[pk_col(m0 - 1, m0)]), R(0), zero);
+      DEp[n + n - 1] = R(0);
+      DEp[2 * n + n - 1] = R(0);
     }
     __syncthreads();
   }
 }
 
 template <class E>
-size_t tridiag_big_lds_bytes() {
-  using R = typename real_of<E>::type;
+size_t tridiag_big_lds_bytes(int n, int k0) {
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  return 3 * al(kNBig * sizeof(E)) + 2 * al(kNBig * sizeof(R)) + al(kNT / kWave * sizeof(R)) +
-         al(kNT / kWave * sizeof(E));
+  const size_t m = (size_t)(n - k0);
+  return al(m * (m + 1) / 2 * sizeof(E)) + 2 * al(m * sizeof(E));
 }
 
 // ===========================================================================
@@ -2408,10 +2487,13 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
 
 template <class E>
 int launch_tridiag_big(const Plan& pl, GramArgs ga, hipStream_t st) {
-  const size_t lds = tridiag_big_lds_bytes<E>();
+  const int k0 = tridiag_big_k0(pl.n, sizeof(E));
+  const size_t lds = tridiag_big_lds_bytes<E>(pl.n, k0);
+  if (k0 > pl.n - 2 || lds + 256 > kTBLds)
+    return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes (k0 %d)", lds, k0);
   auto kern = tridiag_big_kernel<E>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<grid_for(ga.np, 2), kNT, lds, st>>>(ga);
+  kern<<<grid_for(ga.np, 1), kTBT, lds, st>>>(ga, k0);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }

@@ -157,7 +157,7 @@ def config4(seed=4, n_volumes=8, scale=1.0):
     return vols
 
 
-def config4_calls(vols, max_n=128):
+def config4_calls(vols, max_n=512):
     """Functional-API calls for the batched workload (``config4()`` volumes):
     mp_pca / hybrid_pca (noise_std = the sigma map) / optimal_thresholding /
     raw_svt (threshold = tau), patch p, overlap p - 1, the volume's mask.

@@ -35,7 +35,7 @@ def test_batch_matches_single_calls():
 
     calls, skipped = _calls()
     print(f"[batch] {len(calls)} volumes, skipped {skipped}", flush=True)
-    assert len(calls) >= 4
+    assert len(calls) == 8 and not skipped  # includes the 7^3 x 200 volume (n = 200)
     ref = _single(calls)
     out = denoise_batch(calls, use_graph=False)
     for k, ((o, w, _, _), (ro, rw, _, _)) in enumerate(zip(out, ref)):
@@ -68,6 +68,6 @@ def test_batch_reports_unsupported_volume():
 
     rng = np.random.default_rng(0)
     ok = rng.standard_normal((8, 8, 8, 30)).astype(np.float32) + 10
-    big = rng.standard_normal((9, 9, 9, 300)).astype(np.float32) + 10  # n = min(216, 300)
+    big = rng.standard_normal((10, 10, 10, 600)).astype(np.float32) + 10  # n = min(729, 600)
     with pytest.raises(N.NativeError, match="volume 1"):
-        denoise_batch([("mp_pca", (ok, 3, 2), {}), ("mp_pca", (big, 6, 5), {})])
+        denoise_batch([("mp_pca", (ok, 3, 2), {}), ("mp_pca", (big, 9, 8), {})])
