@@ -833,17 +833,39 @@ size_t gram_big_lds_bytes(int M, int T) {
 // reflector from column k, p = tau A v, w = p - tau/2 (p^H v) v and
 // A -= v w^H + w v^H on the trailing block; d, e, e^2 and tau go straight to
 // DE / TAU and v_k to row k of the global block (columns k+1 .. n-1).
+__device__ __forceinline__ float shfl_xor_e(float v, int m) { return shfl_xor_(v, m); }
+__device__ __forceinline__ double shfl_xor_e(double v, int m) { return shfl_xor_(v, m); }
+template <class R> __device__ __forceinline__ cpx<R> shfl_xor_e(cpx<R> v, int m) {
+  return {shfl_xor_(v.x, m), shfl_xor_(v.y, m)};
+}
+
 constexpr int kTBT = 512;                 // threads of the large-Gram tridiagonalisation
 constexpr size_t kTBLds = 160 * 1024;     // its LDS
 __host__ __device__ __forceinline__ int pk_col(int j, int m) { return j * m - (j * (j - 1)) / 2; }
 
-// Smallest k0 with the packed (n - k0)-block, v and w within the LDS.
-inline int tridiag_big_k0(int n, size_t esz) {
+// Smallest k0 with the packed (n - k0)-block, v and w within 1 / per_cu of
+// the LDS.
+inline int tridiag_big_k0(int n, size_t esz, int per_cu = 1) {
   for (int k0 = 0; k0 < n; ++k0) {
     const size_t m = (size_t)(n - k0);
-    if ((m * (m + 1) / 2 + 2 * m + 8) * esz + 256 <= kTBLds) return k0;
+    if (((m * (m + 1) / 2 + 2 * m + 8) * esz + 256) * per_cu <= kTBLds) return k0;
   }
   return n;
+}
+
+// Sum over the block for the large-Gram tridiagonalisation: one barrier; each
+// call site owns its `red` (so no trailing barrier protects its reuse: the
+// next use of the same array is several barriers later).
+template <int NT, class T>
+__device__ __forceinline__ T block_sum1(T v, T* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) red[wid] = v;
+  lds_barrier();
+  T s = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / kWave; ++w) s += red[w];
+  return s;
 }
 
 template <class E>
@@ -963,7 +985,7 @@ __global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
       const R dk = re_(colk[kk]);
       R part = R(0);
       for (int i = kk + 2 + tid; i < m0; i += NT) part += abs2_(colk[i]);
-      const R xn2 = block_sum<NT>(part, redr);
+      const R xn2 = block_sum1<NT>(part, redr);
       const R ar = re_(alpha), ai = im_(alpha);
       E* rowk = A + (size_t)k * n + k0;  // global row k, local columns
       if (!((xn2 > R(0)) || (ai != R(0)))) {
@@ -983,35 +1005,80 @@ __global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
       }
       if (tid == 0) put_step(k, dk, beta, tk);
       __syncthreads();
-      // p_c = tau (sum_{kk < r < c} B[c][r] v_r + sum_{r >= c} conj(B[r][c]) v_r)
-      E pcv = zero;
-      const int c = kk + 1 + tid;
-      if (c < m0) {
-        E sacc = zero;
-        int idx = pk_col(kk + 1, m0) + (c - kk - 1);  // B[c][kk+1]
-        for (int r = kk + 1; r < c; ++r) {
-          sacc += Bp[idx] * v[r];
-          idx += m0 - r - 1;  // B[c][r] -> B[c][r+1]
+      // p_c = tau (sum_{kk < r < c} B[c][r] v_r + sum_{r >= c} conj(B[r][c]) v_r),
+      // the r range of column c split over S adjacent lanes (4 loads in flight
+      // per lane), partial sums combined by a fixed butterfly
+      const int L = m0 - kk - 1;  // trailing columns kk+1 .. m0-1
+      int S = 1;
+      while (S < 8 && 2 * S * L <= NT) S *= 2;
+      const int g = tid / S, sub = tid % S;
+      const int c = kk + 1 + g;
+      E pacc = zero;
+      if (g < L) {
+        const int len = (L + S - 1) / S;
+        const int r0 = kk + 1 + sub * len, r1 = min(m0, r0 + len);
+        const int ra = min(r1, c);
+        int r = r0;
+        for (; r + 4 <= ra; r += 4) {
+          E x[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) x[u] = Bp[pk_col(r + u, m0) + (c - r - u)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) pacc += x[u] * v[r + u];
         }
+        for (; r < ra; ++r) pacc += Bp[pk_col(r, m0) + (c - r)] * v[r];
         const E* colc = Bp + pk_col(c, m0) - c;
-        for (int r = c; r < m0; ++r) sacc += conj_(colc[r]) * v[r];
-        pcv = tk * sacc;
+        r = max(r0, c);
+        for (; r + 4 <= r1; r += 4) {
+          E x[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) x[u] = colc[r + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) pacc += conj_(x[u]) * v[r + u];
+        }
+        for (; r < r1; ++r) pacc += conj_(colc[r]) * v[r];
       }
-      E pv = block_sum<NT>(c < m0 ? conj_(pcv) * v[c < m0 ? c : 0] : zero, rede);
+      for (int o = 1; o < S; o <<= 1) pacc += shfl_xor_e(pacc, o);
+      const bool lead = g < L && sub == 0;
+      const E pcv = tk * pacc;
+      const E pv = block_sum1<NT>(lead ? conj_(pcv) * v[c] : zero, rede);
       const E kc = (R(-0.5) * tk) * pv;
-      if (c < m0) w[c] = pcv + kc * v[c];
+      if (lead) w[c] = pcv + kc * v[c];
       __syncthreads();
-      // B -= v w^H + w v^H on the lower triangle of the trailing block; thread t
-      // takes columns kk+1+t and m0-1-t (equal total length)
-      const int L = m0 - kk - 1;
-      if (tid < (L + 1) / 2) {
-        const int ca = kk + 1 + tid, cb = m0 - 1 - tid;
+      // B -= v w^H + w v^H on the lower triangle of the trailing block: column
+      // pair (kk+1+q, m0-1-q) has L + 1 entries in all, split over S2 lanes
+      const int P = (L + 1) / 2;
+      int S2 = 1;
+      while (S2 < 8 && 2 * S2 * P <= NT) S2 *= 2;
+      const int q = tid / S2, sq = tid % S2;
+      if (q < P) {
+        const int ca = kk + 1 + q, cb = m0 - 1 - q;
+        const int la = m0 - ca, lb = cb != ca ? m0 - cb : 0;
+        const int len = (la + lb + S2 - 1) / S2;
+        const int i0 = sq * len, i1 = min(la + lb, i0 + len);
         for (int h = 0; h < 2; ++h) {
+          // part of [i0, i1) inside column h (column a: [0, la), column b: [la, la + lb))
+          const int lo = h ? max(i0, la) : i0, hi = h ? i1 : min(i1, la);
+          if (lo >= hi) continue;
           const int cc = h ? cb : ca;
-          if (h && cb == ca) break;
+          const int rbase = cc - (h ? la : 0);  // row of concatenated index i is rbase + i
           E* colc = Bp + pk_col(cc, m0) - cc;
           const E cw = conj_(w[cc]), cv = conj_(v[cc]);
-          for (int r = cc; r < m0; ++r) colc[r] = rank2_update(colc[r], v[r], cw, w[r], cv);
+          int i = lo;
+          for (; i + 4 <= hi; i += 4) {
+            E x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = colc[rbase + i + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int rr = rbase + i + u;
+              colc[rr] = rank2_update(x[u], v[rr], cw, w[rr], cv);
+            }
+          }
+          for (; i < hi; ++i) {
+            const int rr = rbase + i;
+            colc[rr] = rank2_update(colc[rr], v[rr], cw, w[rr], cv);
+          }
         }
       }
       __syncthreads();
@@ -2487,13 +2554,20 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
 
 template <class E>
 int launch_tridiag_big(const Plan& pl, GramArgs ga, hipStream_t st) {
-  const int k0 = tridiag_big_k0(pl.n, sizeof(E));
+  // two workgroups per CU (one's barriers overlap the other's work) when that
+  // costs at most a few more global-memory steps
+  int per_cu = 1, k0 = tridiag_big_k0(pl.n, sizeof(E));
+  const int k0b = tridiag_big_k0(pl.n, sizeof(E), 2);
+  if (k0b <= k0 + std::max(2, pl.n / 32)) {
+    per_cu = 2;
+    k0 = k0b;
+  }
   const size_t lds = tridiag_big_lds_bytes<E>(pl.n, k0);
-  if (k0 > pl.n - 2 || lds + 256 > kTBLds)
+  if (k0 > pl.n - 2 || (lds + 256) * per_cu > kTBLds)
     return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes (k0 %d)", lds, k0);
   auto kern = tridiag_big_kernel<E>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<grid_for(ga.np, 1), kTBT, lds, st>>>(ga, k0);
+  kern<<<grid_for(ga.np, per_cu), kTBT, lds, st>>>(ga, k0);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }
@@ -2545,16 +2619,24 @@ int launch_eig_all(const Plan& pl, EigArgs ea, hipStream_t st) {
 }
 
 constexpr int kSynSCBig = 8, kSynKGBig = 4;  // synth chunking for Grams above 128
-static_assert(kSynSC * 128 <= kSynNI * kNT && kSynSCBig * kNBig <= kSynNI * kNT,
+static_assert(kSynSC * 128 <= kSynNI * kNT && kSynSCBig * kNBig <= kSynNI * kNT &&
+                  2 * kSynSCBig * 256 <= kSynNI * kNT,
               "one staged synthesis chunk must fit kSynNI items per thread");
 
 template <class IN, class E, bool TSIDE>
 int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) {
   const bool big = pl.tile == 0;
-  const size_t lds = big ? synth_lds_bytes<IN, E, TSIDE, kSynSCBig, kSynKGBig>(pl.n, pl.kmax, pl.g.M, pl.g.pt)
-                         : synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
+  // Grams of 129 .. 256: 16-row / -frame chunks and groups of 8 components
+  // when they fit the LDS, else 8 and 4
+  const size_t lds_mid = synth_lds_bytes<IN, E, TSIDE, 2 * kSynSCBig, 2 * kSynKGBig>(
+      pl.n, pl.kmax, pl.g.M, pl.g.pt);
+  const bool mid = big && pl.n <= 256 && lds_mid <= 160 * 1024;
+  const size_t lds = mid ? lds_mid
+                     : big ? synth_lds_bytes<IN, E, TSIDE, kSynSCBig, kSynKGBig>(pl.n, pl.kmax, pl.g.M, pl.g.pt)
+                           : synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "synth LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
+  auto kern = mid ? synth_kernel<IN, E, TSIDE, 2 * kSynSCBig, 2 * kSynKGBig>
+                  : big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
   kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);
