@@ -221,6 +221,51 @@ def bench_batched(steps):
         speedup_graph_vs_sequential=t_seq / t_graph)
 
 
+def bench_large(steps):
+    """Gram sizes above 128 (global-memory + packed-LDS Householder): the
+    49x49x52x200 float32 volume of the batched workload (p = 7, MP-PCA, Gram
+    200) and the complex64 64x64x40x200 NORDIC workload (p = 11, Gram 200,
+    FP32 eigen stage, noise_std 25), one pdamd_denoise call each, inputs
+    resident in HBM (NORDIC's host threshold draws run once, untimed)."""
+    import ctypes
+
+    import torch
+
+    from patch_denoise import _native as N
+    from patch_denoise import workloads
+    from patch_denoise.batch import BatchPlan
+
+    vol4 = workloads.config4_calls(workloads.config4())[0][4]
+    cases = [("mppca_49x49x52x200_p7_o6_ellipsoid_weighted", vol4),
+             ("nordic_c64_64x64x40x200_p11_o10_fp32",
+              ("nordic", (workloads.config3(), 11, 10), {"noise_std": 25.0, "precision": "fp32"}))]
+    res = {}
+    for name, call in cases:
+        job = BatchPlan([call], use_graph=False).jobs[0]
+        run = N.DeviceRun(job.prob, job.data, job.mask_arr, job.var_map)
+        run.run()
+        torch.cuda.synchronize()
+        ms = (ctypes.c_float * len(N.STAGES))()
+        cnt = (ctypes.c_int64 * len(N.STAGES))()
+        job.prob.stage_ms = ctypes.cast(ms, ctypes.c_void_p).value
+        job.prob.stage_launches = ctypes.cast(cnt, ctypes.c_void_p).value
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.run()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        job.prob.stage_ms = None
+        job.prob.stage_launches = None
+        vf = float(np.prod(job.data.shape))
+        n = min(int(np.prod(job.prob.patch[:3])), int(job.prob.patch[3]))
+        res[name] = dict(gram_size=n, patch=list(job.prob.patch), ms_per_call=t * 1e3,
+                         voxel_frames_s=vf / t, steps=steps,
+                         stage_ms_per_call={nm: ms[i] / steps for i, nm in enumerate(N.STAGES)
+                                            if cnt[i]})
+        del run
+    return res
+
+
 def bench_noise_map(steps):
     """Sliding-window noise map (estimate_noise) on an 80x80x48x300 float32
     volume, block 3, input resident in HBM.  HBM-bound: the algorithmic bytes
@@ -264,7 +309,10 @@ def main():
                     help="steps of the batched multi-volume measurement; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")
-    ap.add_argument("--only", choices=["all", "headline", "differentiable", "batched", "noise"],
+    ap.add_argument("--large-steps", type=int, default=2,
+                    help="steps of the Gram-size-200 measurements; 0 skips them")
+    ap.add_argument("--only", choices=["all", "headline", "differentiable", "batched", "noise",
+                                       "large"],
                     default="all", help="run only one part (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM counter calibration)")
@@ -289,6 +337,9 @@ def main():
         return
     if args.only == "noise":
         print(json.dumps({"noise_map": bench_noise_map(5)}))
+        return
+    if args.only == "large":
+        print(json.dumps({"large_gram": bench_large(max(args.large_steps, 1))}))
         return
     t_gen = time.perf_counter()
     data, mask = workloads.config2()
@@ -382,6 +433,7 @@ def main():
     diff = bench_differentiable(args.diff_steps) if full and args.diff_steps > 0 else None
     batched = bench_batched(args.batch_steps) if full and args.batch_steps > 0 else None
     noise_map = bench_noise_map(5) if full else None
+    large = bench_large(args.large_steps) if full and args.large_steps > 0 else None
 
     split = None
     if args.only == "all" and args.precision == "auto":
@@ -442,6 +494,7 @@ def main():
         "differentiable": diff,
         "batched": batched,
         "noise_map": noise_map,
+        "large_gram": large,
         "datagen_s": gen_s,
     }
     print(json.dumps(line, default=_jsonable))
