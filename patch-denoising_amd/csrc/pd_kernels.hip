@@ -712,7 +712,9 @@ size_t tridiag_lds_bytes(int n) {
 // resident while the workgroup works on it).
 //   gram_big_kernel     per patch: per-frame means (rows added in order), then
 //                       the Gram in 64 x 64 output tiles (lower tiles, each
-//                       mirrored), reduction staged through LDS 16 at a time
+//                       mirrored), reduction staged through LDS 32 at a time;
+//                       FP32 real / complex on MFMA 32x32x2 (a wave per 32 x 32
+//                       quarter; complex = 4 real products), FP64 on the VALU
 //   tridiag_big_kernel  per patch: unblocked Householder (LAPACK ?hetd2,
 //                       lower) on the full Hermitian matrix in global memory,
 //                       one thread per column for the product and the rank-2
@@ -722,7 +724,7 @@ size_t tridiag_lds_bytes(int n) {
 // ---------------------------------------------------------------------------
 constexpr int kNBig = 512;  // largest supported Gram size
 constexpr int kGBT = 64;    // Gram output tile
-constexpr int kGBK = 16;    // reduction indices per staged chunk
+constexpr int kGBK = 32;    // reduction indices per staged chunk
 
 template <class IN, class E, bool TSIDE>
 __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vol, GramArgs a) {
@@ -732,6 +734,10 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   const int K = TSIDE ? M : T;  // reduction length
   const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
   constexpr int LW = kGBT + 1;
+  constexpr bool MF = std::is_same<R, float>::value;  // FP32 real / complex on MFMA
+  constexpr bool CPX = is_cpx<E>::value;
+  const int wv = tid / kWave, lane = tid % kWave;
+  const int wi = wv >> 1, wj = wv & 1, lhi = lane >> 5, llo = lane & 31;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
   IN* mean = lds_carve<IN>(cur, T);
@@ -776,16 +782,45 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
       while ((bi + 1) * (bi + 2) / 2 <= tp) ++bi;
       const int bj = tp - bi * (bi + 1) / 2;
       const int i0 = bi * kGBT, j0 = bj * kGBT;
-      E acc[4][4];
+      E acc[MF ? 1 : 4][MF ? 1 : 4];
+      floatx16 cre, cim;
+      if constexpr (MF) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+        for (int q = 0; q < 16; ++q) cre[q] = cim[q] = 0.0f;
+      } else {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = E(R(0));
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] = E(R(0));
+      }
       for (int k0 = 0; k0 < K; k0 += kGBK) {
         const int kc = min(kGBK, K - k0);
         stage(As, i0, k0, kc);
         stage(Bs, j0, k0, kc);
         lds_barrier();
+        if constexpr (MF) {
+          // rows k >= kc of the staging are zero
+          for (int k = 0; k < kc; k += 2) {
+            const E xa = As[(k + lhi) * LW + 32 * wi + llo];
+            const E xb = Bs[(k + lhi) * LW + 32 * wj + llo];
+            if constexpr (CPX) {
+              const float ar = re_(xa), ai = im_(xa), br = re_(xb), bim = im_(xb);
+              cre = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cre, 0, 0, 0);
+              cre = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bim, cre, 0, 0, 0);
+              if (TSIDE) {  // conj(a) b: im = ar bi - ai br
+                cim = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bim, cim, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, br, cim, 0, 0, 0);
+              } else {  // a conj(b): im = ai br - ar bi
+                cim = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, cim, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f32_32x32x2f32(-ar, bim, cim, 0, 0, 0);
+              }
+            } else {
+              cre = __builtin_amdgcn_mfma_f32_32x32x2f32(re_(xa), re_(xb), cre, 0, 0, 0);
+            }
+          }
+          lds_barrier();
+          continue;
+        }
         for (int k = 0; k < kc; ++k) {
           E xa[4], xb[4];
 #pragma unroll
@@ -802,18 +837,33 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
         }
         lds_barrier();
       }
+      if constexpr (MF) {
+        // 32x32 MFMA output: accumulator q of lane l is C[8(q/4) + 4(l/32) + q%4][l%32]
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = i0 + ty + 16 * u, c = j0 + tx + 16 * v;
+        for (int q = 0; q < 16; ++q) {
+          const int r = i0 + 32 * wi + 8 * (q >> 2) + 4 * lhi + (q & 3);
+          const int c = j0 + 32 * wj + llo;
           if (r < n && c < n) {
-            E x = acc[u][v];
+            E x = mk<E>(cre[q], cim[q]);
             if (a.scale) x = div_n(x, M - 1);
             G[(size_t)r * n + c] = x;
             if (bi != bj) G[(size_t)c * n + r] = conj_(x);
           }
         }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = i0 + ty + 16 * u, c = j0 + tx + 16 * v;
+            if (r < n && c < n) {
+              E x = acc[u][v];
+              if (a.scale) x = div_n(x, M - 1);
+              G[(size_t)r * n + c] = x;
+              if (bi != bj) G[(size_t)c * n + r] = conj_(x);
+            }
+          }
+      }
     }
     lds_barrier();
   }
