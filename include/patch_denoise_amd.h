@@ -42,8 +42,12 @@ enum {
   PDAMD_MPPCA = 0,   /* lowrank.py:42-63   Marchenko-Pastur PCA on the top-10 eigenvalues */
   PDAMD_HYBRID = 1,  /* lowrank.py:104-119 Hybrid-PCA against a prior variance map       */
   PDAMD_HARD = 2,    /* lowrank.py:170-184 hard singular-value threshold (raw SVT, NORDIC) */
-  PDAMD_OPTIMAL = 3  /* lowrank.py:379-407 optimal shrinkage (Gavish-Donoho)             */
+  PDAMD_OPTIMAL = 3, /* lowrank.py:379-407 optimal shrinkage (Gavish-Donoho)             */
+  PDAMD_ADAPTIVE = 4 /* lowrank.py:410-636 adaptive trace-norm shrinkage (SURE/GSURE/QUT) */
 };
+
+/* adaptive parameter selection (lowrank.py:410-540) */
+enum { PDAMD_ADAPT_SURE = 0, PDAMD_ADAPT_GSURE = 1, PDAMD_ADAPT_QUT = 2 };
 
 /* recombination (base.py:243-256) */
 enum { PDAMD_WEIGHTED = 0, PDAMD_AVERAGE = 1, PDAMD_CENTER = 2 };
@@ -126,6 +130,20 @@ typedef struct pdamd_problem {
      synchronisation at the end of the call.                                 */
   float*   stage_ms;
   int64_t* stage_launches;
+
+  /* ADAPTIVE (lowrank.py:410-636; the prior variance map var_map_dev is
+     required: the patch noise level is sqrt(mean variance over its voxels)).
+     SURE / GSURE: tau minimises the SURE / GSURE cost in log(tau) from
+     tau0_log (NaN = log median singular value) for each exponent in
+     gamma_dev[0 .. n_gamma) (n_gamma >= 1); the lowest cost wins.
+     QUT: tau from qut_tau_dev (float64, by linear patch index, computed by
+     the caller from its Monte-Carlo draws); gamma = gamma_dev[0] when
+     n_gamma == 1, else minimises the SURE cost in log(gamma - 1) from 0.   */
+  int    adaptive_method;  /* PDAMD_ADAPT_* */
+  int    n_gamma;
+  const double* gamma_dev;
+  double tau0_log;
+  const double* qut_tau_dev;
 } pdamd_problem;
 
 /* stages for pdamd_problem.stage_ms */

@@ -469,3 +469,102 @@ def estimate_noise(noise_sequence, block_size=1):
         win = tuple(slice(ci, ci + b) for ci in c)
         out[tuple(ci + b // 2 for ci in c)] = np.std(noise_sequence[win])
     return out
+
+
+# --------------------------------------------------------------------------
+# Adaptive trace-norm shrinkage (space_time/lowrank.py:410-636)
+# --------------------------------------------------------------------------
+
+
+def atn_shrink(s, gamma, tau):
+    """s_i max(1 - (tau / s_i)^gamma, 0) (lowrank.py:459-461)."""
+    return s * np.maximum(1 - (tau / s) ** gamma, 0)
+
+
+def atn_cost(shape, method, s, gamma, sigma=None, tau=None):
+    """SURE / GSURE cost of the adaptive trace-norm estimator (lowrank.py:410-442).
+
+    ``shape`` = (rows, frames) of the Casorati matrix.  The free variable is
+    log(tau) for "sure" / "gsure" and log(gamma - 1) for "qut".  The pair
+    matrix D is held in float32 as in the reference (NumPy path).
+    """
+    nr, nc = shape
+    if method == "qut":
+        gamma = np.exp(gamma) + 1
+    else:
+        tau = np.exp(tau)
+    sq = s**2
+    k = len(s)
+    dhat = atn_shrink(s, gamma, tau)
+    num = s * dhat
+    pair = np.zeros((k, k), dtype=np.float32)
+    for i in range(k):
+        den = sq[i] - sq
+        den[i] = np.inf
+        pair[i, :] = num[i] / den
+    gradd = (1 + (gamma - 1) * (tau / s) ** gamma) * (s >= tau)
+    div = np.sum(gradd + abs(nr - nc) * dhat / s) + 2 * np.sum(pair)
+    rss = np.sum((dhat - s) ** 2)
+    if method == "gsure":
+        return rss / (1 - div / nr / nc) ** 2
+    return (sigma**2) * ((-nr * nc) + (2 * div)) + rss
+
+
+def adaptive_params(shape, s, stdest, method, gamma0, tau0, nbsim):
+    """(gamma, tau) of lowrank.py:444-513: QUT = quantile of nbsim Monte-Carlo
+    top singular values (global np.random) and, unless gamma0 is a float,
+    gamma by BFGS on the cost; SURE / GSURE = BFGS on log(tau) from tau0
+    (log median singular value) for each gamma candidate, lowest cost wins."""
+    from scipy.optimize import minimize
+
+    if method == "qut":
+        maxd = np.ones(nbsim)
+        for i in range(nbsim):
+            maxd[i] = np.max(svd(np.random.randn(*shape) * stdest, compute_uv=False,
+                                 overwrite_a=True))
+        tau = np.quantile(maxd, 1 - 1 / np.sqrt(np.log(max(*shape))))
+        if isinstance(gamma0, (float, np.floating)):
+            return gamma0, tau
+        res = minimize(lambda g: atn_cost(shape, "qut", s, g, stdest, tau), 0)
+        return np.exp(res.x) + 1, tau
+    if tau0 is None:
+        tau0 = np.log(np.median(s))
+    best = np.inf
+    gamma = tau = None
+    for g in gamma0:
+        res = minimize(lambda x: atn_cost(shape, method, s, g, stdest, x), tau0)  # noqa: B023
+        c = atn_cost(shape, method, s, g, stdest, res.x)
+        if c < best:
+            gamma, tau, best = g, np.exp(res.x), c
+    return gamma, tau
+
+
+def adaptive_patch(X, var_patch, method, gamma0, tau0, nbsim):
+    """Adaptive shrinkage of one Casorati matrix (lowrank.py:609-636)."""
+    stdest = np.sqrt(np.mean(var_patch))
+    U, s, Vt, m = svd_analysis(X)
+    gamma, tau = adaptive_params(X.shape, s, stdest, method, gamma0, tau0, nbsim)
+    sh = atn_shrink(s, gamma, tau)
+    if np.any(sh):
+        r = np.max(np.nonzero(sh)) + 1
+        out = svd_synthesis(U, sh, Vt, m, r)
+    else:
+        r = 0
+        out = np.zeros_like(X) + m
+    return out, r, np.nan
+
+
+def adaptive_thresholding(input_data, patch_shape, patch_overlap, mask=None, mask_threshold=50,
+                          recombination="weighted", method="SURE", nbsim=500, tau0=None,
+                          gamma0=None, noise_std=1.0, progbar=None):
+    """denoise.py:284-348 / AdaptiveDenoiser (lowrank.py:543-595)."""
+    if recombination not in ("weighted", "average", "center"):
+        raise ValueError("recombination must be one of 'weighted', 'average', 'center'")
+    if method.lower() not in ("sure", "qut", "gsure"):
+        raise ValueError(f"Unsupported method: '{method}'")
+    ps, po = patch_params(patch_shape, patch_overlap, input_data.shape)
+    prior = _prior_variance(noise_std, input_data.shape, ps, po)
+    meth = method.lower()
+    return denoise(input_data, ps, po,
+                   lambda X, i: adaptive_patch(X, prior.get(i), meth, gamma0, tau0, nbsim),
+                   recombination, mask, mask_threshold)

@@ -1162,6 +1162,10 @@ struct EigArgs {
   int rule_f64;     // the reference's spectrum dtype is float64 (f64/c128 data)
   int scale_f64, tau_f64;
   double ts2, tau, mpm, beta;
+  int adaptive_method, n_gamma;  // ADAPTIVE: PDAMD_ADAPT_*, candidates in gamma
+  const double* gamma;
+  double tau0_log;
+  const double* qut_tau;         // QUT: tau by linear patch index
   const double* var_map;
   const double* LAM;  // optimal rule: per-linear-patch scale of the noise level c (null = 1)
   const void* H;
@@ -1394,6 +1398,108 @@ __device__ double shrink(int loss, double y, double beta) {
   return sqrt(fmax(t * t - 4.0 * beta, 0.0) / y);
 }
 
+// Adaptive trace-norm shrinkage (lowrank.py:410-636) on the singular values
+// sv[0..n) of one patch (rows nr, frames nc): dhat_i = s_i max(1 - (tau /
+// s_i)^gamma, 0) and the SURE / GSURE cost of lowrank.py:410-442,
+//   div  = sum_i [(1 + (gamma - 1)(tau/s_i)^gamma) [s_i >= tau] + |nr - nc| dhat_i / s_i]
+//          + 2 sum_{i != j} s_i dhat_i / (s_i^2 - s_j^2)
+//   rss  = sum_i (dhat_i - s_i)^2
+//   SURE = sig2 (-nr nc + 2 div) + rss,   GSURE = rss / (1 - div / (nr nc))^2.
+// Wave-parallel (every lane returns the same bits): lane-strided terms, the
+// pair sum only over components with dhat_i != 0.  Zero singular values (a
+// rank-deficient centred patch) contribute nothing.  Computed in float64.
+__device__ double atn_cost(const double* __restrict__ sv, int n, double gamma, double tau,
+                           bool gsure, double sig2, double nr, double nc) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double part = 0.0, rss = 0.0, dsum = 0.0;
+  for (int i = lane; i < n; i += kWave) {
+    const double si = sv[i];
+    if (!(si > 0.0)) continue;
+    const double pw = pow(tau / si, gamma);
+    const double dh = si * fmax(1.0 - pw, 0.0);
+    part += (si >= tau ? 1.0 + (gamma - 1.0) * pw : 0.0) + fabs(nr - nc) * dh / si;
+    rss += (dh - si) * (dh - si);
+    const double tmp = si * dh;
+    if (tmp != 0.0) {
+      const double si2 = si * si;
+      for (int j = 0; j < n; ++j)
+        if (j != i && sv[j] > 0.0) dsum += tmp / (si2 - sv[j] * sv[j]);
+    }
+  }
+  part = dpp_sum(part);
+  rss = dpp_sum(rss);
+  dsum = dpp_sum(dsum);
+  const double div = part + 2.0 * dsum;
+  if (gsure) {
+    const double q = 1.0 - div / nr / nc;
+    return rss / (q * q);
+  }
+  return sig2 * (-nr * nc + 2.0 * div) + rss;
+}
+
+// Local minimum of f (wave-uniform) reached from x0: downhill bracketing with
+// growing steps, then golden-section search to |interval| <= 1e-10 (1 + |x|).
+// NaN costs count as +inf.  The point with the lowest cost seen is returned.
+template <class F>
+__device__ double local_min(F f, double x0, double* fbest) {
+  auto g = [&](double x) {
+    const double v = f(x);
+    return isnan(v) ? INFINITY : v;
+  };
+  double a = x0, fa = g(a), h = 0.25;
+  double b = a + h, fb = g(b);
+  if (fb > fa) {  // downhill is the other way, or x0 is already bracketed
+    const double bm = a - h, fbm = g(bm);
+    if (fbm >= fa) {
+      // [a - h, a + h] brackets a minimum near x0
+      double lo = bm, hi = b;
+      double bx = a, bf = fa;
+      const double gr = 0.6180339887498949;
+      double x1 = hi - gr * (hi - lo), x2 = lo + gr * (hi - lo);
+      double f1 = g(x1), f2 = g(x2);
+      for (int it = 0; it < 200 && (hi - lo) > 1e-10 * (1.0 + fabs(bx)); ++it) {
+        if (f1 <= f2) {
+          hi = x2; x2 = x1; f2 = f1; x1 = hi - gr * (hi - lo); f1 = g(x1);
+        } else {
+          lo = x1; x1 = x2; f1 = f2; x2 = lo + gr * (hi - lo); f2 = g(x2);
+        }
+        if (f1 < bf) { bf = f1; bx = x1; }
+        if (f2 < bf) { bf = f2; bx = x2; }
+      }
+      *fbest = bf;
+      return bx;
+    }
+    b = bm;
+    fb = fbm;
+    h = -h;
+  }
+  // fb <= fa: walk downhill with doubling steps until the cost rises
+  double c = b + h, fc = g(c);
+  for (int it = 0; it < 60 && fc <= fb; ++it) {
+    a = b; fa = fb;
+    b = c; fb = fc;
+    h *= 2.0;
+    c = b + h;
+    fc = g(c);
+  }
+  double lo = fmin(a, c), hi = fmax(a, c);
+  double bx = b, bf = fb;
+  const double gr = 0.6180339887498949;
+  double x1 = hi - gr * (hi - lo), x2 = lo + gr * (hi - lo);
+  double f1 = g(x1), f2 = g(x2);
+  for (int it = 0; it < 200 && (hi - lo) > 1e-10 * (1.0 + fabs(bx)); ++it) {
+    if (f1 <= f2) {
+      hi = x2; x2 = x1; f2 = f1; x1 = hi - gr * (hi - lo); f1 = g(x1);
+    } else {
+      lo = x1; x1 = x2; f1 = f2; x2 = lo + gr * (hi - lo); f2 = g(x2);
+    }
+    if (f1 < bf) { bf = f1; bx = x1; }
+    if (f2 < bf) { bf = f2; bx = x2; }
+  }
+  *fbest = bf;
+  return bx;
+}
+
 // Every eigenvector of the tridiagonal (d, e), eigenvalues lam[0..n) in
 // ascending order, into Z (row stride ks, column c = vector c).  Lane c of the
 // wavefront handles component c (64 at a time) with a twisted factorization
@@ -1528,6 +1634,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   R* lam = lds_carve<R>(cur, n + 16);       // eigenvalue of each component
   int* cidx = lds_carve<int>(cur, n + 16);  // ascending eigen index of each component
   double* wc = lds_carve<double>(cur, n + 16);
+  double* sv = lds_carve<double>(cur, n + 16);  // ADAPTIVE: singular values, descending
   R* zb = lds_carve<R>(cur, (size_t)n * KB);  // z, component-major per lane: zb[i * KB + c]
   __shared__ int sh_i[4];
   __shared__ double sh_d[4];
@@ -1654,6 +1761,91 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           lam[c] = s.val[c];
           wc[c] = 1.0;
         }
+        sh_i[0] = ncomp;
+        sh_i[1] = rank;
+      }
+      wave_sync();
+      ncomp = sh_i[0];
+      rank = sh_i[1];
+    } else if (a.method == PDAMD_ADAPTIVE) {
+      // ---- adaptive trace-norm shrinkage, lowrank.py:597-636 (parameters: 444-540)
+      for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
+      wave_sync();
+      multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
+      // singular values, descending, rounded to the data's precision (the
+      // reference's SVD returns them in that precision)
+      for (int j = lane; j < n; j += kWave)
+        sv[j] = (double)(RT)sqrt(fmax((double)s.val[n - 1 - j], 0.0));
+      wave_sync();
+      const double sig = sqrt(prior), sig2 = sig * sig;
+      const double nr = (double)a.g.M, ncl = (double)a.g.pt;
+      double gam, tau;
+      if (a.adaptive_method == PDAMD_ADAPT_QUT) {
+        tau = a.qut_tau[a.idx[p]];
+        if (a.n_gamma >= 1) {
+          gam = a.gamma[0];
+        } else {  // lowrank.py:478-490: gamma = exp(y) + 1, y from 0
+          double fb;
+          const double y = local_min(
+              [&](double yy) { return atn_cost(sv, n, exp(yy) + 1.0, tau, false, sig2, nr, ncl); },
+              0.0, &fb);
+          gam = exp(y) + 1.0;
+        }
+      } else {  // lowrank.py:494-513: each gamma, tau from log(tau0)
+        const bool gs = a.adaptive_method == PDAMD_ADAPT_GSURE;
+        const bool x32 = isnan(a.tau0_log);  // default start: float32 / float64 log median
+        double x0 = a.tau0_log;
+        if (x32) {  // log(median(s)) in the singular values' precision
+          const RT md = (n % 2) ? (RT)sv[n / 2] : (RT)(((RT)sv[n / 2 - 1] + (RT)sv[n / 2]) / RT(2));
+          x0 = (double)(RT)log(md);
+        }
+        double best = INFINITY;
+        gam = a.gamma[0];
+        tau = exp(x0);
+        if (!a.rule_f64) {
+          // Single-precision data: the reference's cost is evaluated in float32
+          // and its BFGS (finite-difference gradients) returns the start point
+          // itself, so tau = exp(x0) (a float32 exponential for the default
+          // start) and gamma = the first candidate of least cost there.
+          const double tq = x32 ? (double)(RT)exp((RT)x0) : exp(x0);
+          for (int q = 0; q < a.n_gamma; ++q) {
+            const double fq = atn_cost(sv, n, a.gamma[q], tq, gs, sig2, nr, ncl);
+            if (fq < best) {
+              best = fq;
+              gam = a.gamma[q];
+            }
+          }
+          tau = tq;
+        } else {
+          for (int q = 0; q < a.n_gamma; ++q) {
+            const double gq = a.gamma[q];
+            double fq;
+            const double xq = local_min(
+                [&](double x) { return atn_cost(sv, n, gq, exp(x), gs, sig2, nr, ncl); }, x0, &fq);
+            if (fq < best) {
+              best = fq;
+              gam = gq;
+              tau = exp(xq);
+            }
+          }
+        }
+      }
+      if (lane == 0) {
+        int nk = 0, last = -1;
+        for (int j = 0; j < n; ++j) {  // descending position j
+          const double sj = sv[j];
+          const double dh = sj > 0.0 ? sj * fmax(1.0 - pow(tau / sj, gam), 0.0) : 0.0;
+          if (dh != 0.0) {
+            last = j;
+            const int ai = n - 1 - j;  // ascending eigen index
+            cidx[nk] = ai;
+            lam[nk] = s.val[ai];
+            wc[nk] = dh / sqrt(fmax((double)s.val[ai], 0.0));
+            ++nk;
+          }
+        }
+        ncomp = nk;
+        rank = last + 1;
         sh_i[0] = ncomp;
         sh_i[1] = rank;
       }
@@ -1922,7 +2114,7 @@ size_t trieig_lds_bytes(int n, int kb) {
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
   size_t s = 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int));
   s += al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(R));
-  s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(double));
+  s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + 2 * al((n + 16) * sizeof(double));
   s += al((size_t)n * kb * sizeof(R));
   return s;
 }
@@ -2390,7 +2582,14 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
     if (P->shape[i] < 1 || P->patch[i] < 1 || P->patch[i] > P->shape[i] || P->step[i] < 0)
       return fail(PDAMD_EINVAL, "bad shape/patch/step in dim %d", i);
   }
-  if (P->method < 0 || P->method > 3) return fail(PDAMD_EINVAL, "bad method %d", P->method);
+  if (P->method < 0 || P->method > PDAMD_ADAPTIVE) return fail(PDAMD_EINVAL, "bad method %d", P->method);
+  if (P->method == PDAMD_ADAPTIVE) {
+    if (P->adaptive_method < 0 || P->adaptive_method > PDAMD_ADAPT_QUT)
+      return fail(PDAMD_EINVAL, "bad adaptive method %d", P->adaptive_method);
+    if (P->adaptive_method != PDAMD_ADAPT_QUT && (P->n_gamma < 1 || !P->gamma_dev))
+      return fail(PDAMD_EINVAL, "SURE / GSURE need at least one gamma candidate");
+    if (P->n_gamma > 0 && !P->gamma_dev) return fail(PDAMD_EINVAL, "gamma_dev missing");
+  }
   if (P->recombination < 0 || P->recombination > 2)
     return fail(PDAMD_EINVAL, "recombination must be one of 'weighted', 'average', 'center'");
   Geom& g = pl->g;
@@ -2434,7 +2633,8 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   if (P->precision == PDAMD_PREC_FP32 || P->precision == PDAMD_PREC_BF16X3)
     e64 = in64;  // never below the input precision
   else if (P->precision == PDAMD_PREC_FP64) e64 = true;
-  else e64 = in64 || (P->method == PDAMD_HARD || P->method == PDAMD_OPTIMAL);
+  else e64 = in64 || (P->method == PDAMD_HARD || P->method == PDAMD_OPTIMAL ||
+                      P->method == PDAMD_ADAPTIVE);
   if (cplx && n > 64 && e64 && !in64) {
     // complex double Gram of n > 64 does not fit LDS with this kernel
   }
@@ -2713,6 +2913,11 @@ EigArgs eig_args(const pdamd_problem* P, const Plan& pl, char* ws, const GramArg
   ea.beta = (double)pl.g.pt / (double)pl.g.M;
   ea.var_map = P->var_map_dev;
   ea.LAM = P->lam_dev;
+  ea.adaptive_method = P->adaptive_method;
+  ea.n_gamma = P->n_gamma;
+  ea.gamma = P->gamma_dev;
+  ea.tau0_log = P->tau0_log;
+  ea.qut_tau = P->qut_tau_dev;
   ea.H = ga.H;
   ea.DE = ga.DE;
   ea.TAU = ga.TAU;
@@ -2998,8 +3203,10 @@ int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes,
     return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, pl.total);
   if (!P->input_dev || !P->output_dev || !P->weights_dev)
     return fail(PDAMD_EINVAL, "null data pointer");
-  if ((P->method == PDAMD_HYBRID) && !P->var_map_dev)
-    return fail(PDAMD_EINVAL, "hybrid rule needs a variance map");
+  if ((P->method == PDAMD_HYBRID || P->method == PDAMD_ADAPTIVE) && !P->var_map_dev)
+    return fail(PDAMD_EINVAL, "hybrid / adaptive rules need a variance map");
+  if (P->method == PDAMD_ADAPTIVE && P->adaptive_method == PDAMD_ADAPT_QUT && !P->qut_tau_dev)
+    return fail(PDAMD_EINVAL, "QUT needs the per-patch threshold array");
   if (P->lam_dev && P->method != PDAMD_OPTIMAL)
     return fail(PDAMD_EINVAL, "a threshold scale map is only used by the optimal rule");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -3053,8 +3260,10 @@ int pdamd_denoise_batch(pdamd_problem* probs, int count, void* workspace_dev,
     if (rc) return fail(rc, "volume %d: %s", i, g_err.c_str());
     if (!P->input_dev || !P->output_dev || !P->weights_dev)
       return fail(PDAMD_EINVAL, "volume %d: null data pointer", i);
-    if (P->method == PDAMD_HYBRID && !P->var_map_dev)
-      return fail(PDAMD_EINVAL, "volume %d: hybrid rule needs a variance map", i);
+    if ((P->method == PDAMD_HYBRID || P->method == PDAMD_ADAPTIVE) && !P->var_map_dev)
+      return fail(PDAMD_EINVAL, "volume %d: hybrid / adaptive rules need a variance map", i);
+    if (P->method == PDAMD_ADAPTIVE && P->adaptive_method == PDAMD_ADAPT_QUT && !P->qut_tau_dev)
+      return fail(PDAMD_EINVAL, "volume %d: QUT needs the per-patch threshold array", i);
     if (P->lam_dev && P->method != PDAMD_OPTIMAL)
       return fail(PDAMD_EINVAL, "volume %d: a threshold scale map is only used by the optimal rule", i);
     off[i] = tot;

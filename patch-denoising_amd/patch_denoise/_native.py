@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libpdamd.so")
 
 F32, F64, C64, C128 = 0, 1, 2, 3
-MPPCA, HYBRID, HARD, OPTIMAL = 0, 1, 2, 3
+MPPCA, HYBRID, HARD, OPTIMAL, ADAPTIVE = 0, 1, 2, 3, 4
+ADAPT = {"sure": 0, "gsure": 1, "qut": 2}
 WEIGHTED, AVERAGE, CENTER = 0, 1, 2
 LOSS = {"fro": 0, "nuc": 1, "ope": 2}
 RECOMB = {"weighted": WEIGHTED, "average": AVERAGE, "center": CENTER}
@@ -62,6 +63,11 @@ class Problem(ctypes.Structure):
         ("lam_dev", ctypes.c_void_p),
         ("stage_ms", ctypes.c_void_p),
         ("stage_launches", ctypes.c_void_p),
+        ("adaptive_method", ctypes.c_int),
+        ("n_gamma", ctypes.c_int),
+        ("gamma_dev", ctypes.c_void_p),
+        ("tau0_log", ctypes.c_double),
+        ("qut_tau_dev", ctypes.c_void_p),
     ]
 
 
@@ -177,6 +183,7 @@ def make_problem(shape4, patch4, step4, dtype, method, recombination="weighted",
     p.precision = PREC[precision]
     p.mask_kind = MASK_NONE
     p.mask_threshold = 50.0
+    p.tau0_log = float("nan")
     return p
 
 

@@ -24,7 +24,8 @@ from .space_time import base as _base
 
 __all__ = ["denoise_batch", "BatchPlan"]
 
-_FUNCS = ("mp_pca", "hybrid_pca", "raw_svt", "nordic", "optimal_thresholding")
+_FUNCS = ("mp_pca", "hybrid_pca", "raw_svt", "nordic", "optimal_thresholding",
+          "adaptive_thresholding")
 
 
 def _collect(calls):

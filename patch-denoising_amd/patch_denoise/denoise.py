@@ -7,6 +7,7 @@ reference.  ``precision`` is an extra keyword-only option ("auto", "fp32",
 """
 
 from .space_time.lowrank import (
+    AdaptiveDenoiser,
     HybridPCADenoiser,
     MPPCADenoiser,
     NordicDenoiser,
@@ -64,3 +65,13 @@ def optimal_thresholding(input_data, patch_shape, patch_overlap, mask=None, mask
     return denoiser.denoise(input_data, mask=mask, noise_std=noise_std,
                             mask_threshold=mask_threshold,
                             eps_marshenko_pastur=eps_marshenko_pastur, progbar=progbar)
+
+
+def adaptive_thresholding(input_data, patch_shape, patch_overlap, mask=None, mask_threshold=50,
+                          recombination="weighted", method="SURE", nbsim=500, tau0=None,
+                          gamma0=None, noise_std=1.0, progbar=None, *, precision="auto"):
+    """Adaptive trace-norm shrinkage, SURE / GSURE / QUT (reference denoise.py:284-348)."""
+    denoiser = AdaptiveDenoiser(patch_shape, patch_overlap, recombination=recombination,
+                                method=method, nbsim=nbsim, precision=precision)
+    return denoiser.denoise(input_data, mask, mask_threshold, tau0, noise_std, gamma0,
+                            progbar=progbar)

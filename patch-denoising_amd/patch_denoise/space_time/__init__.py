@@ -2,6 +2,7 @@
 
 from .base import BaseSpaceTimeDenoiser, PatchedArray
 from .lowrank import (
+    AdaptiveDenoiser,
     HybridPCADenoiser,
     MPPCADenoiser,
     NordicDenoiser,
@@ -17,4 +18,5 @@ __all__ = [
     "NordicDenoiser",
     "OptimalSVDDenoiser",
     "RawSVDDenoiser",
+    "AdaptiveDenoiser",
 ]

@@ -221,3 +221,133 @@ class OptimalSVDDenoiser(BaseSpaceTimeDenoiser):
             patch[None], var_rows=rows,
             configure=lambda prob, p: self._set_rule(prob, mp_median, shrink_func))
         return out[0], int(r[0]), np.nan
+
+
+def _qut_max_svals(n_rows, n_frames, stdest, nbsim):
+    """Largest singular value of each of ``nbsim`` draws ``randn(n_rows,
+    n_frames) * stdest`` from the global legacy ``np.random`` stream, in the
+    reference's order (lowrank.py:446-456: one ``randn(*patch.shape)`` per
+    draw; a single ``randn(nbsim, n_rows, n_frames)`` yields the same
+    numbers).  The values come from the float64 min-side Gram's top
+    eigenvalue, on the GPU when present (same value as the SVD to ~1e-15)."""
+    a = np.random.randn(nbsim, n_rows, n_frames) * stdest
+    try:
+        import torch
+    except ImportError:  # pragma: no cover - torch is part of the runtime
+        torch = None
+    if torch is not None and torch.cuda.is_available():
+        d = torch.from_numpy(a).cuda()
+        g = d.transpose(1, 2).contiguous() @ d if n_frames <= n_rows else d @ d.transpose(1, 2).contiguous()
+        top = torch.linalg.eigvalsh(g)[:, -1]
+        return torch.sqrt(torch.clamp(top, min=0)).cpu().numpy()
+    return np.array([max(svd(x, compute_uv=False)) for x in a])
+
+
+class AdaptiveDenoiser(BaseSpaceTimeDenoiser):
+    """Adaptive trace-norm shrinkage, SURE / GSURE / QUT parameter selection
+    (reference: lowrank.py:543-636, after Josse & Sardy).
+
+    The per-patch parameter search runs on the GPU: SURE / GSURE minimise the
+    reference's cost in log(tau) for each gamma candidate from log(tau0)
+    (log median singular value by default) by downhill bracketing and
+    golden-section search; QUT takes tau from the same Monte-Carlo draws as
+    the reference (global ``np.random``, patch order) and, unless a float
+    gamma0 is given, minimises the cost in log(gamma - 1) from 0.  The
+    reference minimises with BFGS on finite-difference gradients of a cost
+    whose pair matrix is rounded to float32, so parameters agree to the
+    optimiser's precision, not bit for bit.
+    """
+
+    _rule = N.ADAPTIVE
+    _SUPPORTED_METHOD = ["sure", "qut", "gsure"]
+
+    def __init__(self, patch_shape, patch_overlap, method="SURE", recombination="weighted",
+                 nbsim=500, **kwargs):
+        super().__init__(patch_shape, patch_overlap, recombination, **kwargs)
+        if method.lower() not in self._SUPPORTED_METHOD:
+            raise ValueError(
+                f"Unsupported method: '{method}', use any of {self._SUPPORTED_METHOD}"
+            )
+        self.input_denoising_kwargs["method"] = method.lower()
+        self.input_denoising_kwargs["nbsim"] = nbsim
+
+    def denoise(self, input_data, mask=None, mask_threshold=50, tau0=None, noise_std=None,
+                gamma0=None, progbar=None):
+        input_data = np.asarray(input_data)
+        self.input_denoising_kwargs["gamma0"] = gamma0
+        self.input_denoising_kwargs["tau0"] = tau0
+        p_s, p_o = self._get_patch_param(input_data.shape)
+        if isinstance(noise_std, (float, np.floating)):
+            var_apriori = noise_std**2 * np.ones(input_data.shape[:-1])
+        else:
+            var_apriori = noise_std**2
+        full = np.broadcast_to(var_apriori[..., None], input_data.shape)
+        self.input_denoising_kwargs["var_apriori"] = PatchedArray(full, p_s, p_o)
+        self._var_map = np.ascontiguousarray(full[..., 0], dtype=np.float64)
+        self._mask_for_qut = (mask, mask_threshold)
+        return super().denoise(input_data, mask, mask_threshold, progbar=progbar)
+
+    # -- rule operands --------------------------------------------------------------
+    def _gammas(self, method, gamma0):
+        if method == "qut":
+            return [float(gamma0)] if isinstance(gamma0, (float, np.floating)) else []
+        return [float(g) for g in gamma0]  # None raises TypeError, as in the reference
+
+    def _set_rule(self, prob, method, gamma0, tau0, qut_tau=None):
+        import torch
+
+        gam = self._gammas(method, gamma0)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self._gamma_t = torch.tensor(gam if gam else [0.0], dtype=torch.float64, device=dev)
+        prob.adaptive_method = N.ADAPT[method]
+        prob.n_gamma = len(gam)
+        prob.gamma_dev = self._gamma_t.data_ptr()
+        prob.tau0_log = float("nan") if tau0 is None else float(np.asarray(tau0).reshape(-1)[0])
+        if qut_tau is not None:
+            self._qut_t = torch.from_numpy(np.ascontiguousarray(qut_tau, np.float64)).to(dev)
+            prob.qut_tau_dev = self._qut_t.data_ptr()
+
+    def _qut_taus(self, prob, input_data, mask_arr, npatch):
+        """QUT threshold of every selected patch, in patch order (lowrank.py:444-457)."""
+        kw = self.input_denoising_kwargs
+        idx = N.select_patches(prob, mask_arr)
+        var_ap = kw["var_apriori"]
+        taus = np.zeros(max(npatch, 1), np.float64)
+        M = int(np.prod(prob.patch[:3]))
+        T = int(prob.patch[3])
+        q = 1 - 1 / np.sqrt(np.log(max(M, T)))
+        for i in idx:
+            stdest = np.sqrt(np.mean(var_ap.get_patch(int(i))))
+            taus[i] = np.quantile(_qut_max_svals(M, T, stdest, kw["nbsim"]), q)
+        return taus
+
+    def _configure(self, prob, input_data, p_s):
+        kw = self.input_denoising_kwargs
+        qut = None
+        self._set_rule(prob, kw["method"], kw["gamma0"], kw["tau0"])  # valid for the selection
+        if kw["method"] == "qut":
+            npatch = 1
+            for i in range(4):
+                S, p, st = int(prob.shape[i]), int(prob.patch[i]), int(prob.step[i])
+                npatch *= ((S - p) // st + 1) if (p < S and st > 0) else 1
+            mask, _ = self._mask_for_qut
+            mask_arr = None if mask is None else np.asarray(mask)
+            qut = self._qut_taus(prob, input_data, mask_arr, npatch)
+        self._set_rule(prob, kw["method"], kw["gamma0"], kw["tau0"], qut)
+        return self._var_map
+
+    def _patch_processing(self, patch, patch_idx=None, gamma0=None, tau0=None, var_apriori=None,
+                          method=None, nbsim=None):
+        """Adaptive shrinkage of one Casorati matrix, on the GPU (lowrank.py:597-636)."""
+        patch = np.asarray(patch)
+        stdest = np.sqrt(np.mean(var_apriori.get_patch(patch_idx)))
+        rows = np.full((1, patch.shape[0]), stdest * stdest)
+        qut = None
+        if method == "qut":
+            M, T = patch.shape
+            qut = np.array([np.quantile(_qut_max_svals(M, T, stdest, nbsim),
+                                        1 - 1 / np.sqrt(np.log(max(M, T))))])
+        out, r, _, _ = self._process_casorati(
+            patch[None], var_rows=rows,
+            configure=lambda prob, p: self._set_rule(prob, method, gamma0, tau0, qut))
+        return out[0], int(r[0]), np.nan

@@ -22,6 +22,9 @@ Fixtures written (all small, ``np.savez_compressed``):
                        and a seeded sample of output voxel-frames
 * ``noise_map.npz``    sliding-window noise maps (estimate_noise) on small
                        1-D / 2-D / 3-D, float / complex volumes
+* ``adaptive.npz``     adaptive trace-norm shrinkage (SURE / GSURE / QUT):
+                       per-patch (gamma, tau) of the reference's parameter
+                       search and end-to-end outputs on two small volumes
 """
 
 import os
@@ -223,11 +226,67 @@ def noise_map():
     _save("noise_map.npz", **arrays)
 
 
+def adaptive():
+    """Adaptive shrinkage (lowrank.py:410-636, denoise.py:284-348).
+
+    Per patch: singular values, the reference's (gamma, tau) for SURE and
+    GSURE (gamma candidates linspace(1, 5, 10)), QUT with a float gamma and
+    with gamma estimated (global np.random seeded before each patch, nbsim
+    draws); end to end: adaptive_thresholding on a 3-D + T (T-side Grams) and
+    a 2-D + T (M-side) volume.
+    """
+    rng = np.random.default_rng(31)
+    arrays = {}
+    gam = np.linspace(1, 5, 10)
+    mats = []
+    for M, T, rank in ((27, 20, 2), (16, 30, 3), (36, 24, 4), (27, 40, 1)):
+        base = rng.standard_normal((M, rank)) @ rng.standard_normal((rank, T)) * 6
+        mats.append((50 + base + rng.standard_normal((M, T)) * 1.5).astype(np.float32))
+    for k, X in enumerate(mats):
+        _, s, _, _ = RU.svd_analysis(X)
+        arrays[f"patch{k}_X"] = X
+        arrays[f"patch{k}_s"] = s
+        for method in ("sure", "gsure"):
+            g, t = RL._get_gamma_tau(X, s, 1.5, method, gam, None)
+            arrays[f"patch{k}_{method}_gamma"] = np.asarray(g, np.float64)
+            arrays[f"patch{k}_{method}_tau"] = np.asarray(t, np.float64).reshape(-1)
+        for label, g0 in (("qutg2", 2.0), ("qut", None)):
+            np.random.seed(100 + k)
+            g, t = RL._get_gamma_tau_qut(X, s, 1.5, g0, 60)
+            arrays[f"patch{k}_{label}_gamma"] = np.asarray(g, np.float64).reshape(-1)
+            arrays[f"patch{k}_{label}_tau"] = np.asarray(t, np.float64)
+    arrays["gamma_grid"] = gam
+    arrays["stdest"] = np.asarray(1.5)
+    vols = {
+        "v3d": ((50 + rng.standard_normal((7, 7, 6, 20)) * 1.5
+                 + np.tensordot(rng.standard_normal((2, 7, 7, 6)),
+                                rng.standard_normal((2, 20)) * 4, axes=([0], [0])))
+                .astype(np.float32), 3, 1),
+        "v2d": ((50 + rng.standard_normal((9, 8, 30)) * 1.5
+                 + np.tensordot(rng.standard_normal((2, 9, 8)),
+                                rng.standard_normal((2, 30)) * 4, axes=([0], [0])))
+                .astype(np.float32), 4, 2),
+    }
+    for name, (data, ps, po) in vols.items():
+        arrays[f"{name}_in"] = data
+        arrays[f"{name}_ps"] = np.asarray(ps)
+        arrays[f"{name}_po"] = np.asarray(po)
+        for method, g0, recomb in (("SURE", gam, "weighted"), ("GSURE", gam, "average"),
+                                   ("qut", 2.0, "weighted")):
+            np.random.seed(7)
+            out, w, _, _ = R.adaptive_thresholding(data, ps, po, recombination=recomb,
+                                                   method=method, nbsim=40, gamma0=g0,
+                                                   noise_std=1.5)
+            arrays[f"{name}_{method.lower()}_out"] = out
+            arrays[f"{name}_{method.lower()}_w"] = w
+    _save("adaptive.npz", **arrays)
+
+
 if __name__ == "__main__":
     import warnings
 
     warnings.simplefilter("ignore")
-    which = sys.argv[1:] or ["e2e", "patches", "mpm", "tau", "cfg1", "noise"]
+    which = sys.argv[1:] or ["e2e", "patches", "mpm", "tau", "cfg1", "noise", "adaptive"]
     if "mpm" in which:
         mp_median_grid()
     if "tau" in which:
@@ -240,3 +299,5 @@ if __name__ == "__main__":
         cfg1_full()
     if "noise" in which:
         noise_map()
+    if "adaptive" in which:
+        adaptive()

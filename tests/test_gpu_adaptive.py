@@ -1,0 +1,185 @@
+"""GPU parity for adaptive trace-norm shrinkage (SURE / GSURE / QUT).
+
+* End to end against the reference's own outputs (tests/golden/adaptive.npz:
+  3-D + T T-side Grams and 2-D + T M-side Grams, float32): rel-L2 <= 1e-5,
+  identical weights.  On float32 data the reference's BFGS returns its start
+  point (the cost is float32, its finite-difference gradient degenerates), so
+  tau = median singular value and gamma = the least-cost candidate there:
+  the rule is reproduced exactly.
+* QUT with gamma estimated (parity unpinned): the reference's BFGS stops
+  after one iteration with "precision loss" (its finite-difference gradient
+  of a cost with a float32 pair matrix is noise), at the line search's first
+  acceptable trial point (y = +-1.01 or an interpolated step), not at the
+  cost's minimum.  The GPU returns the local minimum downhill from y = 0, so
+  its cost is never above the reference's at the reference's gamma (checked
+  per patch on the host) and the outputs differ by the gamma choice
+  (rel-L2 ~1e-3 here, bound 5e-3).
+* float64 data, SURE / GSURE: the reference's BFGS stops after at most one
+  step ("precision loss"), a point no other optimiser reproduces; the GPU
+  returns the local minimum of the same cost.  Parity unpinned there; checked
+  by the reference's own property test (noise reduced).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel_l2(got, want):
+    fin = np.isfinite(want)
+    d = np.asarray(got, np.float64)[fin] - np.asarray(want, np.float64)[fin]
+    return float(np.linalg.norm(d) / max(np.linalg.norm(np.asarray(want, np.float64)[fin]), 1e-300))
+
+
+def test_adaptive_against_reference_fixtures():
+    import patch_denoise as pd
+
+    z = np.load(os.path.join(GOLDEN, "adaptive.npz"))
+    gam = z["gamma_grid"]
+    for v in ("v3d", "v2d"):
+        data, ps, po = z[f"{v}_in"], int(z[f"{v}_ps"]), int(z[f"{v}_po"])
+        for method, g0, recomb in (("SURE", gam, "weighted"), ("GSURE", gam, "average"),
+                                   ("qut", 2.0, "weighted")):
+            print(f"[adaptive] {v} {method} {recomb}", flush=True)
+            np.random.seed(7)
+            out, w, _, _ = pd.adaptive_thresholding(data, ps, po, recombination=recomb,
+                                                    method=method, nbsim=40, gamma0=g0,
+                                                    noise_std=1.5)
+            want = z[f"{v}_{method.lower()}_out"]
+            assert out.dtype == want.dtype and out.shape == want.shape
+            assert np.array_equal(np.isnan(out), np.isnan(want))
+            e = rel_l2(out, want)
+            ew = rel_l2(w, z[f"{v}_{method.lower()}_w"])
+            print(f"[adaptive]   rel-L2 out {e:.3g} weights {ew:.3g}", flush=True)
+            assert e <= 1e-5 and ew <= 1e-6
+
+
+def _volume(seed, shape=(9, 9, 8, 30), dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    sig = np.tensordot(rng.standard_normal((3,) + shape[:-1]),
+                       rng.standard_normal((3, shape[-1])) * 5, axes=([0], [0]))
+    clean = 80 + sig
+    return clean, (clean + 2.0 * rng.standard_normal(shape)).astype(dtype)
+
+
+def test_adaptive_qut_gamma_search_against_oracle():
+    import warnings
+
+    import patch_denoise as pd
+    from oracle import cpu_ref as O
+    from scipy.optimize import minimize
+
+    _, data = _volume(3)
+    for ps, po in ((3, 2), (4, 2)):
+        print(f"[adaptive] qut gamma search patch {ps}", flush=True)
+        np.random.seed(11)
+        out, w, _, _ = pd.adaptive_thresholding(data, ps, po, method="qut", nbsim=30,
+                                                noise_std=2.0)
+        np.random.seed(11)
+        want, want_w, _, _ = O.adaptive_thresholding(data, ps, po, method="qut", nbsim=30,
+                                                     noise_std=2.0)
+        e = rel_l2(out, want)
+        print(f"[adaptive]   rel-L2 out {e:.3g}", flush=True)
+        assert np.array_equal(np.isnan(out), np.isnan(want))
+        assert e <= 5e-3 and rel_l2(w, want_w) <= 1e-6
+    # per patch: the GPU's gamma (from its single-patch plugin call) has a cost
+    # no higher than the reference optimiser's endpoint
+    pp, po4 = O.patch_params(3, 2, data.shape)
+    src = O.StridedPatches(data, pp, po4)
+    var = O._prior_variance(2.0, data.shape, pp, po4)
+    den = pd.AdaptiveDenoiser(3, 2, method="qut", nbsim=30)
+    for i in range(4):
+        X = src.get(i).reshape(27, -1)
+        _, s, _, _ = O.svd_analysis(X)
+        np.random.seed(40 + i)
+        got, r_gpu, _ = den._patch_processing(X, patch_idx=i, gamma0=None, tau0=None,
+                                              var_apriori=_Pv(var), method="qut", nbsim=30)
+        np.random.seed(40 + i)
+        maxd = np.array([np.max(O.svd(np.random.randn(*X.shape) * 2.0, compute_uv=False))
+                         for _ in range(30)])
+        tau = np.quantile(maxd, 1 - 1 / np.sqrt(np.log(max(*X.shape))))
+        f = lambda y: O.atn_cost(X.shape, "qut", s, y, 2.0, tau)  # noqa: E731
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            y_ref = minimize(f, 0).x[0]
+            ys = np.linspace(-3, 3, 6001)
+            fs = np.array([f(y) for y in ys])
+        y_best = ys[np.nanargmin(fs)]
+        # the GPU output equals the oracle patch at its own best gamma
+        U, s2, Vt, m = O.svd_analysis(X)
+        best_out = None
+        for y in (y_best,):
+            sh = O.atn_shrink(s2, np.exp(y) + 1, tau)
+            r = np.max(np.nonzero(sh)) + 1 if np.any(sh) else 0
+            best_out = O.svd_synthesis(U, sh, Vt, m, r) if r else np.zeros_like(X) + m
+        e = rel_l2(got, best_out)
+        print(f"[adaptive]   patch {i}: reference y {y_ref:.3f} cost {f(y_ref):.6g}, "
+              f"grid min y {y_best:.3f} cost {np.nanmin(fs):.6g}; GPU vs grid-min output {e:.2g}",
+              flush=True)
+        assert np.nanmin(fs) <= f(y_ref) + 1e-9 * abs(f(y_ref))
+        assert e <= 1e-3
+
+
+class _Pv:
+    """Stand-in for the reference's variance PatchedArray (get_patch by index)."""
+
+    def __init__(self, strided):
+        self._s = strided
+
+    def get_patch(self, i):
+        return self._s.get(i)
+
+
+def test_adaptive_masked_and_float32_sure_against_oracle():
+    import patch_denoise as pd
+    from oracle import cpu_ref as O
+
+    _, data = _volume(5, shape=(10, 9, 7, 24))
+    mask = np.ones(data.shape[:-1], bool)
+    mask[:2] = False
+    sigma_map = 2.0 * (1 + 0.2 * np.random.default_rng(1).random(data.shape[:-1]))
+    for method, recomb in (("SURE", "weighted"), ("GSURE", "average")):
+        kw = dict(mask=mask, recombination=recomb, method=method,
+                  gamma0=np.linspace(1, 4, 7), noise_std=sigma_map)
+        out, w, _, _ = pd.adaptive_thresholding(data, 3, 1, **kw)
+        want, want_w, _, _ = O.adaptive_thresholding(data, 3, 1, **kw)
+        e = rel_l2(out, want)
+        print(f"[adaptive] masked {method} {recomb}: rel-L2 {e:.3g}", flush=True)
+        assert np.array_equal(np.isnan(out), np.isnan(want))
+        assert e <= 1e-5 and rel_l2(w, want_w) <= 1e-6
+
+
+@pytest.mark.parametrize("method, gamma", [("qut", None), ("gsure", np.linspace(1, 5, 10)),
+                                           ("sure", np.linspace(1, 5, 10))])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_adaptive_reduces_noise(method, gamma, dtype):
+    """The reference's own check (tests/test_denoiser.py:115-136)."""
+    import patch_denoise as pd
+
+    clean, noisy = _volume(9, shape=(12, 12, 10, 40), dtype=dtype)
+    np.random.seed(0)
+    den, _, _, _ = pd.adaptive_thresholding(noisy, 4, 0, method=method, gamma0=gamma,
+                                            nbsim=50, noise_std=2.0)
+    before = np.sqrt(np.nanmean(np.nanvar(noisy - clean, axis=-1)))
+    after = np.sqrt(np.nanmean(np.nanvar(den - clean, axis=-1)))
+    print(f"[adaptive] {method} {np.dtype(dtype).name}: noise {before:.3f} -> {after:.3f}",
+          flush=True)
+    assert np.isfinite(den[:, :, :8]).all()  # z = 8, 9: no patch covers them (0 / 0, as the reference)
+    assert after < before
+
+
+def test_adaptive_errors_match_reference():
+    import patch_denoise as pd
+
+    _, data = _volume(2, shape=(6, 6, 6, 20))
+    with pytest.raises(ValueError):
+        pd.AdaptiveDenoiser(3, 2, method="bogus")
+    with pytest.raises(TypeError):  # SURE needs an iterable of gamma candidates
+        pd.adaptive_thresholding(data, 3, 2, method="SURE", gamma0=None)
+    with pytest.raises(TypeError):  # AdaptiveDenoiser.denoise: noise_std None is squared
+        pd.AdaptiveDenoiser(3, 2, method="SURE").denoise(data, gamma0=[1.0, 2.0])

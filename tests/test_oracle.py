@@ -175,3 +175,69 @@ def test_shrinkers_closed_forms():
     with np.errstate(invalid="ignore", divide="ignore"):
         n = O.shrink_nuc(np.array([1.0, 3.0]), b)
     assert np.isnan(n[0]) and n[1] > 0
+
+
+def test_adaptive_parameters_match_reference_fixtures():
+    """(gamma, tau) of the oracle's SURE / GSURE / QUT search against the
+    reference's own on four Casorati matrices (tests/golden/adaptive.npz)."""
+    import warnings
+
+    z = np.load(os.path.join(GOLDEN, "adaptive.npz"))
+    gam = z["gamma_grid"]
+    sd = float(z["stdest"])
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for k in range(4):
+            X, s = z[f"patch{k}_X"], z[f"patch{k}_s"]
+            for m in ("sure", "gsure"):
+                g, t = O.adaptive_params(X.shape, s, sd, m, gam, None, 0)
+                assert g == z[f"patch{k}_{m}_gamma"]
+                np.testing.assert_allclose(t, z[f"patch{k}_{m}_tau"], rtol=1e-7)
+            for label, g0 in (("qutg2", 2.0), ("qut", None)):
+                np.random.seed(100 + k)
+                g, t = O.adaptive_params(X.shape, s, sd, "qut", g0, None, 60)
+                np.testing.assert_allclose(np.ravel(g), z[f"patch{k}_{label}_gamma"], rtol=1e-7)
+                np.testing.assert_allclose(t, z[f"patch{k}_{label}_tau"], rtol=1e-12)
+
+
+def test_adaptive_end_to_end_matches_reference_fixtures():
+    import warnings
+
+    z = np.load(os.path.join(GOLDEN, "adaptive.npz"))
+    gam = z["gamma_grid"]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for v in ("v3d", "v2d"):
+            data, ps, po = z[f"{v}_in"], int(z[f"{v}_ps"]), int(z[f"{v}_po"])
+            for method, g0, recomb in (("SURE", gam, "weighted"), ("GSURE", gam, "average"),
+                                       ("qut", 2.0, "weighted")):
+                np.random.seed(7)
+                out, w, _, _ = O.adaptive_thresholding(data, ps, po, recombination=recomb,
+                                                       method=method, nbsim=40, gamma0=g0,
+                                                       noise_std=1.5)
+                want = z[f"{v}_{method.lower()}_out"]
+                assert np.array_equal(np.isnan(out), np.isnan(want))
+                fin = np.isfinite(want)
+                np.testing.assert_allclose(out[fin], want[fin], rtol=0, atol=1e-5 * np.abs(want[fin]).max())
+                np.testing.assert_array_equal(w, z[f"{v}_{method.lower()}_w"])
+
+
+def test_adaptive_cost_limits_closed_form():
+    """SURE at tau -> 0 is the unbiased risk of X itself, sigma^2 n p (the
+    divergence of the identity is n p: k^2 + k |n - p| = k max(n, p) with
+    k = min(n, p)); at tau -> infinity it is the risk of 0, |X|^2 - sigma^2 n p.
+    A dropped pair term, a wrong |n - p| factor or sign breaks one of them."""
+    rng = np.random.default_rng(8)
+    for nr, nc in ((12, 7), (7, 12), (9, 9)):
+        s = np.sort(rng.uniform(1.0, 10.0, min(nr, nc)))[::-1].astype(np.float64)
+        sig = 0.7
+        lo = O.atn_cost((nr, nc), "sure", s, 2.0, sig, np.log(1e-9))
+        hi = O.atn_cost((nr, nc), "sure", s, 2.0, sig, np.log(1e9))
+        np.testing.assert_allclose(lo, sig**2 * nr * nc, rtol=1e-5)
+        np.testing.assert_allclose(hi, np.sum(s**2) - sig**2 * nr * nc, rtol=1e-9)
+        # GSURE at tau -> infinity: rss / 1
+        np.testing.assert_allclose(O.atn_cost((nr, nc), "gsure", s, 2.0, sig, np.log(1e9)),
+                                   np.sum(s**2), rtol=1e-9)
+        # shrinker: hard zero below tau, s - tau at gamma = 1 above
+        sh = O.atn_shrink(s, 1.0, 4.0)
+        np.testing.assert_allclose(sh, np.maximum(s - 4.0, 0), rtol=1e-12)
