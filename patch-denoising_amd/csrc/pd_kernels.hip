@@ -1617,7 +1617,7 @@ constexpr int kKBBig = 4;  // the same for Grams above 128 (more rows per lane)
 // clusters, and back-transformation y = H_0 ... H_{n-2} z with the vectors
 // held row-distributed in registers (lane l: rows l + 64 s, s < NR; batches
 // of up to KBC components).
-template <class E, class RT, int NR = 2, int KBC = kKB>
+template <class E, class RT, int NR = 2, int KBC = kKB, bool AD = false>
 __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   using R = typename real_of<E>::type;
   const int lane = threadIdx.x;
@@ -1634,7 +1634,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   R* lam = lds_carve<R>(cur, n + 16);       // eigenvalue of each component
   int* cidx = lds_carve<int>(cur, n + 16);  // ascending eigen index of each component
   double* wc = lds_carve<double>(cur, n + 16);
-  double* sv = lds_carve<double>(cur, n + 16);  // ADAPTIVE: singular values, descending
+  double* sv = AD ? lds_carve<double>(cur, n + 16) : nullptr;  // ADAPTIVE: singular values, descending
   R* zb = lds_carve<R>(cur, (size_t)n * KB);  // z, component-major per lane: zb[i * KB + c]
   __shared__ int sh_i[4];
   __shared__ double sh_d[4];
@@ -1768,7 +1768,10 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       ncomp = sh_i[0];
       rank = sh_i[1];
     } else if (a.method == PDAMD_ADAPTIVE) {
-      // ---- adaptive trace-norm shrinkage, lowrank.py:597-636 (parameters: 444-540)
+      // ---- adaptive trace-norm shrinkage, lowrank.py:597-636 (parameters: 444-540);
+      // compiled only into the AD instantiation (its search code would cost the
+      // other rules' kernel registers)
+      if constexpr (AD) {
       for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
       wave_sync();
       multisect(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
@@ -1852,6 +1855,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       wave_sync();
       ncomp = sh_i[0];
       rank = sh_i[1];
+      }
     } else {
       // ---- optimal shrinkage, lowrank.py:387-407
       double sigma;
@@ -2109,12 +2113,13 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
 }
 
 template <class E>
-size_t trieig_lds_bytes(int n, int kb) {
+size_t trieig_lds_bytes(int n, int kb, bool ad = false) {
   using R = typename real_of<E>::type;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
   size_t s = 3 * al(kWave * sizeof(R)) + al(kWave * sizeof(int));
   s += al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(R));
-  s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) + 2 * al((n + 16) * sizeof(double));
+  s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) +
+       (ad ? 2 : 1) * al((n + 16) * sizeof(double));
   s += al((size_t)n * kb * sizeof(R));
   return s;
 }
@@ -2847,11 +2852,14 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
 
 template <class E, class RT>
 int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
-  const size_t lds = trieig_lds_bytes<E>(pl.n, pl.kb);
+  const bool ad = ea.method == PDAMD_ADAPTIVE;
+  const size_t lds = trieig_lds_bytes<E>(pl.n, pl.kb, ad);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "eigen LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = pl.tile ? trieig_kernel<E, RT>
-                      : (pl.n <= 4 * kWave ? trieig_kernel<E, RT, 4, kKBBig>
-                                           : trieig_kernel<E, RT, kNBig / kWave, kKBBig>);
+  auto kern = pl.tile ? (ad ? trieig_kernel<E, RT, 2, kKB, true> : trieig_kernel<E, RT>)
+              : pl.n <= 4 * kWave
+                  ? (ad ? trieig_kernel<E, RT, 4, kKBBig, true> : trieig_kernel<E, RT, 4, kKBBig>)
+                  : (ad ? trieig_kernel<E, RT, kNBig / kWave, kKBBig, true>
+                        : trieig_kernel<E, RT, kNBig / kWave, kKBBig>);
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = std::max(1, std::min(16, (int)((160 * 1024) / lds)));
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
