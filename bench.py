@@ -212,13 +212,21 @@ def bench_batched(steps):
     t_seq = timed(lambda: [r.run() for r in singles])
     t_eager = timed(eager.run)
     t_graph = timed(graph.run)
+    del graph
+    t0 = time.perf_counter()
+    tuned = BatchPlan(calls, use_graph=True, autotune=True)
+    tune_s = time.perf_counter() - t0
+    t_tuned = timed(tuned.run)
     return dict(workload="batched_8vol_seed4", volumes=[
         dict(method=c[0], shape=list(c[1][0].shape), patch=c[1][1]) for c in calls],
         skipped=[dict(index=i, reason=r) for i, r in skipped], voxel_frames=vf,
         patches=eager.batch.selected, steps=steps, datagen_s=gen_s,
         sequential_ms=t_seq * 1e3, batch_ms=t_eager * 1e3, batch_graph_ms=t_graph * 1e3,
         sequential_voxel_frames_s=vf / t_seq, batch_graph_voxel_frames_s=vf / t_graph,
-        speedup_graph_vs_sequential=t_seq / t_graph)
+        speedup_graph_vs_sequential=t_seq / t_graph,
+        autotuned_graph_ms=t_tuned * 1e3, autotune_s=tune_s,
+        autotuned_launch_per_cu=[list(t) for t in tuned.tuned],
+        speedup_autotuned_vs_sequential=t_seq / t_tuned)
 
 
 def bench_large(steps):

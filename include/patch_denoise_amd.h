@@ -144,6 +144,14 @@ typedef struct pdamd_problem {
   const double* gamma_dev;
   double tau0_log;
   const double* qut_tau_dev;
+
+  /* launch shape: workgroups per CU for the gram, tridiagonalisation, eigen
+     and synthesis launches (0 = the library's default; capped by what the
+     kernel's LDS allows; 1 .. 32).  Set by an autotuner (the Python
+     package's patch_denoise.autotune); results do not depend on it beyond
+     the order of the output's float atomic additions.  No counterpart in
+     the reference.                                                          */
+  int launch_per_cu[4];
 } pdamd_problem;
 
 /* stages for pdamd_problem.stage_ms */

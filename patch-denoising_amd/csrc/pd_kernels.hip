@@ -2565,6 +2565,7 @@ struct Plan {
   Geom g;
   int64_t npatch, nvox;
   int n, kmax, kb, tile;
+  int per_cu[4] = {0, 0, 0, 0};  // launch overrides: gram, tridiag, eig, synth (0 = default)
   bool tside;
   int etype;  // 0 float, 1 double, 2 cpx<float>, 3 cpx<double>
   int rule_f64;
@@ -2629,6 +2630,11 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   if ((P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) && pt < 10)
     return fail(PDAMD_EINVAL, "the eigen rules need at least 10 frames per patch");
   pl->tile = n <= 32 ? 2 : (n <= 64 ? 4 : (n <= 128 ? 8 : 0));  // 0: large-Gram kernels
+  for (int i = 0; i < 4; ++i) {
+    if (P->launch_per_cu[i] < 0 || P->launch_per_cu[i] > 32)
+      return fail(PDAMD_EINVAL, "launch_per_cu[%d] = %d outside 0..32", i, P->launch_per_cu[i]);
+    pl->per_cu[i] = P->launch_per_cu[i];
+  }
   const bool cplx = P->dtype == PDAMD_C64 || P->dtype == PDAMD_C128;
   const bool in64 = P->dtype == PDAMD_F64 || P->dtype == PDAMD_C128;
   pl->rule_f64 = in64 ? 1 : 0;
@@ -2759,6 +2765,12 @@ int zero_async(void* p, size_t bytes, hipStream_t st) {
   return PDAMD_OK;
 }
 
+// Workgroups per CU: the caller's override (autotuner) when set, capped by LDS.
+inline int pick_per_cu(int override_, int dflt, size_t lds) {
+  const int cap = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
+  return std::max(1, std::min(override_ > 0 ? override_ : dflt, cap));
+}
+
 int grid_for(int64_t np, int per_cu) {
   static int n_cu = 0;
   if (!n_cu) {
@@ -2776,7 +2788,7 @@ int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = gram_kernel<IN, E, TSIDE, TILE, BF>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+  const int per_cu = pick_per_cu(pl.per_cu[0], 8, lds);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -2789,7 +2801,7 @@ int launch_tridiag(const Plan& pl, GramArgs ga, hipStream_t st) {
     return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = tridiag_kernel<E, RJ>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+  const int per_cu = pick_per_cu(pl.per_cu[1], 8, lds);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -2801,7 +2813,7 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = gram_big_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, std::min(4, (int)((160 * 1024) / lds)));
+  const int per_cu = pick_per_cu(pl.per_cu[0], 4, lds);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -2861,7 +2873,7 @@ int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
                   : (ad ? trieig_kernel<E, RT, kNBig / kWave, kKBBig, true>
                         : trieig_kernel<E, RT, kNBig / kWave, kKBBig>);
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, std::min(16, (int)((160 * 1024) / lds)));
+  const int per_cu = pick_per_cu(pl.per_cu[2], 16, lds);
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -2896,7 +2908,7 @@ int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) 
   auto kern = mid ? synth_kernel<IN, E, TSIDE, 2 * kSynSCBig, 2 * kSynKGBig>
                   : big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+  const int per_cu = pick_per_cu(pl.per_cu[3], 8, lds);
   kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;

@@ -68,6 +68,7 @@ class Problem(ctypes.Structure):
         ("gamma_dev", ctypes.c_void_p),
         ("tau0_log", ctypes.c_double),
         ("qut_tau_dev", ctypes.c_void_p),
+        ("launch_per_cu", ctypes.c_int * 4),
     ]
 
 

@@ -46,10 +46,16 @@ def _collect(calls):
 class BatchPlan:
     """Prepared batch: device buffers allocated once, reusable with
     :meth:`run` (the second and later runs replay the captured graph when
-    ``use_graph``).  Inputs stay as given at construction."""
+    ``use_graph``).  Inputs stay as given at construction.  ``autotune``
+    chooses each volume's launch shapes first (``patch_denoise.autotune``)."""
 
-    def __init__(self, calls, use_graph=True):
+    def __init__(self, calls, use_graph=True, autotune=False):
         self.jobs = _collect(list(calls))
+        self.tuned = None
+        if autotune:  # launch shapes per volume, before any graph is captured
+            from . import autotune as _tune
+
+            self.tuned = [_tune.tune(j.prob, j.data, j.mask_arr, j.var_map)[0] for j in self.jobs]
         self.batch = N.BatchRun([(j.prob, j.data, j.mask_arr, j.var_map) for j in self.jobs],
                                 use_graph=use_graph)
 

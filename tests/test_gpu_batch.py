@@ -62,6 +62,26 @@ def test_batch_graph_replay():
         print(f"[batch] replay {rep}: ok", flush=True)
 
 
+def test_autotuned_batch_matches_single_calls():
+    """Launch shapes chosen by patch_denoise.autotune: same results (only the
+    atomic summation order of overlapping patches may differ)."""
+    from patch_denoise import autotune
+    from patch_denoise.batch import BatchPlan
+
+    calls, _ = _calls()
+    ref = _single(calls)
+    autotune.clear_cache()
+    plan = BatchPlan(calls, use_graph=True, autotune=True)
+    print(f"[batch] autotuned launch_per_cu: {plan.tuned}", flush=True)
+    assert all(all(0 <= v <= 32 for v in t) for t in plan.tuned)  # 0: library default
+    for rep in range(2):
+        res = plan.run().results()
+        for k, ((o, w, _, _), (ro, rw, _, _)) in enumerate(zip(res, ref)):
+            e, ew = rel_l2(o, ro), rel_l2(w, rw)
+            assert e < 1e-6 and ew < 1e-7, (rep, k, e, ew)
+    print("[batch] autotuned replays: ok", flush=True)
+
+
 def test_batch_reports_unsupported_volume():
     from patch_denoise import _native as N
     from patch_denoise.batch import denoise_batch
