@@ -244,10 +244,14 @@ def bench_large(steps):
     from patch_denoise.batch import BatchPlan
 
     vol4 = workloads.config4_calls(workloads.config4())[0][4]
+    c3 = workloads.config3()
     cases = [("mppca_49x49x52x200_p7_o6_ellipsoid_weighted", vol4),
              ("nordic_c64_64x64x40x200_p11_o10_fp32",
-              ("nordic", (workloads.config3(), 11, 10), {"noise_std": 25.0, "precision": "fp32"}))]
+              ("nordic", (c3, 11, 10), {"noise_std": 25.0, "precision": "fp32"})),
+             ("nordic_c64_64x64x40x200_p11_o10_bf16x3",
+              ("nordic", (c3, 11, 10), {"noise_std": 25.0, "precision": "bf16x3"}))]
     res = {}
+    outs = {}
     for name, call in cases:
         job = BatchPlan([call], use_graph=False).jobs[0]
         run = N.DeviceRun(job.prob, job.data, job.mask_arr, job.var_map)
@@ -270,7 +274,15 @@ def bench_large(steps):
                          voxel_frames_s=vf / t, steps=steps,
                          stage_ms_per_call={nm: ms[i] / steps for i, nm in enumerate(N.STAGES)
                                             if cnt[i]})
+        if name.startswith("nordic"):
+            outs[name] = run.out.clone()
         del run
+    a, b = (outs["nordic_c64_64x64x40x200_p11_o10_bf16x3"],
+            outs["nordic_c64_64x64x40x200_p11_o10_fp32"])
+    fin = torch.isfinite(b)
+    res["nordic_c64_64x64x40x200_p11_o10_bf16x3"]["rel_l2_vs_fp32"] = float(
+        (a - b)[fin].abs().pow(2).sum().sqrt() / b[fin].abs().pow(2).sum().sqrt())
+    del outs
     return res
 
 

@@ -726,7 +726,17 @@ constexpr int kNBig = 512;  // largest supported Gram size
 constexpr int kGBT = 64;    // Gram output tile
 constexpr int kGBK = 32;    // reduction indices per staged chunk
 
-template <class IN, class E, bool TSIDE>
+// Split-BF16 staging row (kGBK indices + pad: 80 bytes, 16-byte aligned rows).
+constexpr int kGBW = kGBK + 8;
+typedef short short8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v) {
+  return __builtin_bit_cast(bf16x8, __builtin_bit_cast(short8, v) ^ (short8)(short)0x8000);
+}
+
+// BF: split-BF16 three-product Gram (each centred value x = hi + lo, hi =
+// bf16(x), lo = bf16(x - hi); sum hi hi + hi lo + lo hi on MFMA 32x32x16 bf16
+// with FP32 accumulation; complex as four real products), FP32 data only.
+template <class IN, class E, bool TSIDE, bool BF = false>
 __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vol, GramArgs a) {
   using R = typename real_of<E>::type;
   const Geom& g = a.g;
@@ -736,14 +746,21 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   constexpr int LW = kGBT + 1;
   constexpr bool MF = std::is_same<R, float>::value;  // FP32 real / complex on MFMA
   constexpr bool CPX = is_cpx<E>::value;
+  constexpr bool BFM = BF && MF;
+  constexpr int NPL = CPX ? 2 : 1;  // real planes per operand (re, im)
   const int wv = tid / kWave, lane = tid % kWave;
   const int wi = wv >> 1, wj = wv & 1, lhi = lane >> 5, llo = lane & 31;
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
   IN* mean = lds_carve<IN>(cur, T);
   int64_t* roff = lds_carve<int64_t>(cur, M);
-  E* As = lds_carve<E>(cur, (size_t)kGBK * LW);  // As[k][i]
-  E* Bs = lds_carve<E>(cur, (size_t)kGBK * LW);
+  E* As = BFM ? nullptr : lds_carve<E>(cur, (size_t)kGBK * LW);  // As[k][i]
+  E* Bs = BFM ? nullptr : lds_carve<E>(cur, (size_t)kGBK * LW);
+  // split-BF16 planes [operand A/B][re/im][hi/lo], each kGBT rows of kGBW
+  __bf16* Pl = BFM ? lds_carve<__bf16>(cur, (size_t)2 * NPL * 2 * kGBT * kGBW) : nullptr;
+  auto plane = [&](int op, int ri, int hl) {
+    return Pl + ((size_t)((op * NPL + ri) * 2 + hl)) * kGBT * kGBW;
+  };
   const int nb = (n + kGBT - 1) / kGBT;
   const int ntile = nb * (nb + 1) / 2;
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
@@ -777,6 +794,31 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
         S[k * LW + i] = v;
       }
     };
+    // split-BF16 staging of operand op (0: rows b0.., 1: columns b0..)
+    auto stage_bf = [&](int op, int b0, int k0, int kc) {
+      for (int q = tid; q < kGBK * kGBT; q += kNT) {
+        int i, k;
+        if (TSIDE) {
+          k = q / kGBT;
+          i = q % kGBT;
+        } else {
+          i = q / kGBK;
+          k = q % kGBK;
+        }
+        E v = E(R(0));
+        if (k < kc && b0 + i < n) {
+          const int rr = TSIDE ? k0 + k : b0 + i, tt = TSIDE ? b0 + i : k0 + k;
+          v = cvt<E>(X[roff[rr] + tt] - mean[tt]);
+        }
+        const float comp[2] = {(float)re_(v), (float)im_(v)};
+#pragma unroll
+        for (int ri = 0; ri < NPL; ++ri) {
+          const __bf16 hi = (__bf16)comp[ri];
+          plane(op, ri, 0)[i * kGBW + k] = hi;
+          plane(op, ri, 1)[i * kGBW + k] = (__bf16)(comp[ri] - (float)hi);
+        }
+      }
+    };
     for (int tp = 0; tp < ntile; ++tp) {
       int bi = 0;
       while ((bi + 1) * (bi + 2) / 2 <= tp) ++bi;
@@ -795,9 +837,45 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
       }
       for (int k0 = 0; k0 < K; k0 += kGBK) {
         const int kc = min(kGBK, K - k0);
-        stage(As, i0, k0, kc);
-        stage(Bs, j0, k0, kc);
-        lds_barrier();
+        if constexpr (BFM) {
+          stage_bf(0, i0, k0, kc);
+          stage_bf(1, j0, k0, kc);
+          lds_barrier();
+          auto ld = [&](int op, int ri, int hl, int row, int k) {
+            return *reinterpret_cast<const bf16x8*>(plane(op, ri, hl) + row * kGBW + k + 8 * lhi);
+          };
+          // x y on 3 products: xh yh + xh yl + xl yh
+          auto prod3 = [&](floatx16 c, bf16x8 xh, bf16x8 xl, bf16x8 yh, bf16x8 yl) {
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, c, 0, 0, 0);
+            return __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, c, 0, 0, 0);
+          };
+          const int ra = 32 * wi + llo, rb = 32 * wj + llo;
+#pragma unroll
+          for (int k = 0; k < kGBK; k += 16) {  // indices >= kc are zero in the planes
+            const bf16x8 arh = ld(0, 0, 0, ra, k), arl = ld(0, 0, 1, ra, k);
+            const bf16x8 brh = ld(1, 0, 0, rb, k), brl = ld(1, 0, 1, rb, k);
+            cre = prod3(cre, arh, arl, brh, brl);
+            if constexpr (CPX) {
+              const bf16x8 aih = ld(0, 1, 0, ra, k), ail = ld(0, 1, 1, ra, k);
+              const bf16x8 bih = ld(1, 1, 0, rb, k), bil = ld(1, 1, 1, rb, k);
+              cre = prod3(cre, aih, ail, bih, bil);
+              if (TSIDE) {  // conj(a) b: im = ar bi - ai br
+                cim = prod3(cim, arh, arl, bih, bil);
+                cim = prod3(cim, neg_bf16x8(aih), neg_bf16x8(ail), brh, brl);
+              } else {  // a conj(b): im = ai br - ar bi
+                cim = prod3(cim, aih, ail, brh, brl);
+                cim = prod3(cim, neg_bf16x8(arh), neg_bf16x8(arl), bih, bil);
+              }
+            }
+          }
+          lds_barrier();
+          continue;
+        } else {
+          stage(As, i0, k0, kc);
+          stage(Bs, j0, k0, kc);
+          lds_barrier();
+        }
         if constexpr (MF) {
           // rows k >= kc of the staging are zero
           for (int k = 0; k < kc; k += 2) {
@@ -869,11 +947,12 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   }
 }
 
-template <class IN, class E>
+template <class IN, class E, bool BF = false>
 size_t gram_big_lds_bytes(int M, int T) {
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  return al((size_t)T * sizeof(IN)) + al((size_t)M * sizeof(int64_t)) +
-         2 * al((size_t)kGBK * (kGBT + 1) * sizeof(E));
+  const size_t st = BF ? al((size_t)2 * (is_cpx<E>::value ? 2 : 1) * 2 * kGBT * kGBW * 2)
+                       : 2 * al((size_t)kGBK * (kGBT + 1) * sizeof(E));
+  return al((size_t)T * sizeof(IN)) + al((size_t)M * sizeof(int64_t)) + st;
 }
 
 // Large-Gram Householder: steps k < k0 on the full matrix in global memory,
@@ -2650,7 +2729,9 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
     // complex double Gram of n > 64 does not fit LDS with this kernel
   }
   pl->etype = (cplx ? 2 : 0) + (e64 ? 1 : 0);
-  pl->bf16x3 = P->precision == PDAMD_PREC_BF16X3 && !e64 && !cplx;
+  // split-BF16: FP32 real M-side Grams of 65..128 (gram_kernel) and every
+  // FP32 real / complex Gram above 128 (gram_big_kernel)
+  pl->bf16x3 = P->precision == PDAMD_PREC_BF16X3 && !e64 && (!cplx || n > 128);
   pl->rbytes = e64 ? 8 : 4;
   pl->ebytes = pl->rbytes * (cplx ? 2 : 1);
   pl->kmax = (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) ? std::min(10, n) : n;
@@ -2809,9 +2890,12 @@ int launch_tridiag(const Plan& pl, GramArgs ga, hipStream_t st) {
 
 template <class IN, class E, bool TSIDE>
 int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
-  const size_t lds = gram_big_lds_bytes<IN, E>(pl.g.M, pl.g.pt);
+  constexpr bool F32 = std::is_same<typename real_of<E>::type, float>::value;
+  const bool bf = F32 && pl.bf16x3;
+  const size_t lds = bf ? gram_big_lds_bytes<IN, E, true>(pl.g.M, pl.g.pt)
+                        : gram_big_lds_bytes<IN, E>(pl.g.M, pl.g.pt);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
-  auto kern = gram_big_kernel<IN, E, TSIDE>;
+  auto kern = bf ? gram_big_kernel<IN, E, TSIDE, F32> : gram_big_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = pick_per_cu(pl.per_cu[0], 4, lds);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);

@@ -98,3 +98,26 @@ def test_large_gram_matches_oracle(case):
     tol = 5e-7 if data.dtype in (np.float64, np.complex128) else 1e-5
     print(f"[large-gram]   rel-L2 out {e:.3g} weights {ew:.3g} (tolerance {tol:g})", flush=True)
     assert e <= tol and ew <= 1e-6
+
+
+@pytest.mark.parametrize("which", ["nordic_c64_p11_T200", "mppca_f32_p7_T200", "mppca_2d_f32_p15_T300"])
+def test_large_gram_split_bf16(which):
+    """precision="bf16x3": the large Gram from split-BF16 three-product MFMA
+    (complex: four real products) with FP32 accumulation; the eigen stage in
+    FP32.  Bound: rel-L2 <= 1e-3 against the oracle (the split drops the
+    lo x lo term, ~2^-16 relative)."""
+    import patch_denoise as pd
+    from oracle import cpu_ref as O
+
+    name, fn, data, (ps, po), kw = next(c for c in CASES if c[0] == which)
+    np.random.seed(0)
+    out, w, _, _ = getattr(pd, fn)(data, ps, po, precision="bf16x3", **kw)
+    np.random.seed(0)
+    ref32, _, _, _ = getattr(pd, fn)(data, ps, po, precision="fp32", **kw)
+    np.random.seed(0)
+    want, want_w, _, _ = getattr(O, fn)(data, ps, po, **kw)
+    e, e32 = rel_l2(out, want), rel_l2(out, ref32)
+    print(f"[large-gram] {name} bf16x3: rel-L2 vs oracle {e:.3g}, vs fp32 path {e32:.3g}",
+          flush=True)
+    assert np.array_equal(np.isnan(out), np.isnan(want))
+    assert e <= 1e-3 and rel_l2(w, want_w) <= 1e-6
