@@ -968,7 +968,7 @@ template <class R> __device__ __forceinline__ cpx<R> shfl_xor_e(cpx<R> v, int m)
   return {shfl_xor_(v.x, m), shfl_xor_(v.y, m)};
 }
 
-constexpr int kTBT = 512;                 // threads of the large-Gram tridiagonalisation
+constexpr int kTBT = 1024;  // threads of the large-Gram tridiagonalisation (512 at 2 per CU)
 constexpr size_t kTBLds = 160 * 1024;     // its LDS
 __host__ __device__ __forceinline__ int pk_col(int j, int m) { return j * m - (j * (j - 1)) / 2; }
 
@@ -997,10 +997,9 @@ __device__ __forceinline__ T block_sum1(T v, T* red) {
   return s;
 }
 
-template <class E>
-__global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
+template <class E, int NT>
+__global__ __launch_bounds__(NT) void tridiag_big_kernel(GramArgs a, int k0) {
   using R = typename real_of<E>::type;
-  constexpr int NT = kTBT;
   const int n = a.n, tid = threadIdx.x;
   const int m0 = n - k0;
   extern __shared__ __align__(16) char smem[];
@@ -1048,9 +1047,9 @@ __global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
       __syncthreads();
       for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = vg[c];
       if (tid == 0) put_step(k, dk, beta, tk);
-      E pc[kNBig / NT];
+      E pc[(kNBig + NT - 1) / NT];
 #pragma unroll
-      for (int j = 0; j < kNBig / NT; ++j) {
+      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
         const int c = k + 1 + tid + j * NT;
         E sacc = zero;
         if (c < n) {
@@ -1069,20 +1068,20 @@ __global__ __launch_bounds__(kTBT) void tridiag_big_kernel(GramArgs a, int k0) {
       }
       E pv = zero;
 #pragma unroll
-      for (int j = 0; j < kNBig / NT; ++j) {
+      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
         const int c = k + 1 + tid + j * NT;
         if (c < n) pv += conj_(pc[j]) * vg[c];
       }
       pv = block_sum<NT>(pv, rede);
       const E kc = (R(-0.5) * tk) * pv;
 #pragma unroll
-      for (int j = 0; j < kNBig / NT; ++j) {
+      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
         const int c = k + 1 + tid + j * NT;
         if (c < n) wg[c] = pc[j] + kc * vg[c];
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kNBig / NT; ++j) {
+      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
         const int c = k + 1 + tid + j * NT;
         if (c < n) {
           E* col = A + c;
@@ -2916,9 +2915,11 @@ int launch_tridiag_big(const Plan& pl, GramArgs ga, hipStream_t st) {
   const size_t lds = tridiag_big_lds_bytes<E>(pl.n, k0);
   if (k0 > pl.n - 2 || (lds + 256) * per_cu > kTBLds)
     return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes (k0 %d)", lds, k0);
-  auto kern = tridiag_big_kernel<E>;
+  // one workgroup per CU: 1024 threads (16 waves hide the per-step barriers);
+  // two per CU: 512 threads each (the register file holds both)
+  auto kern = per_cu == 1 ? tridiag_big_kernel<E, kTBT> : tridiag_big_kernel<E, kTBT / 2>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<grid_for(ga.np, per_cu), kTBT, lds, st>>>(ga, k0);
+  kern<<<grid_for(ga.np, per_cu), per_cu == 1 ? kTBT : kTBT / 2, lds, st>>>(ga, k0);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
 }
