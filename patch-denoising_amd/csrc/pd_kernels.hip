@@ -2308,7 +2308,13 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
   E* B = lds_carve<E>(cur, (size_t)KG * SC);  // M-side: B[c][t]; T-side: C[r][c]
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? SC : M);
 
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  // Patches are visited in a permuted order (it -> it * 1000003 mod np, a
+  // bijection since the prime exceeds any chunk) so that workgroups running at
+  // the same time add into distant parts of the output: neighbouring patches
+  // overlap, and their concurrent float atomics would contend for the same
+  // cache lines.
+  for (int it = blockIdx.x; it < a.np; it += gridDim.x) {
+    const int p = (int)(((int64_t)it * 1000003) % a.np);
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
