@@ -22,6 +22,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -945,6 +946,173 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
     }
     lds_barrier();
   }
+}
+
+// One-pass variant for FP32 real Grams up to 256 and complex up to 224: every
+// 32-index chunk of the centred patch is staged once for the whole Gram, and
+// each of the 8 waves keeps its share of the lower 32 x 32 blocks (block w,
+// w + 8, ...) in MFMA accumulators (real: up to 5 blocks, complex: 4).  The
+// patch is read once instead of once per 64 x 64 tile pair.  BF: split-BF16
+// three products, as gram_big_kernel.
+constexpr int kG1T = 512;  // threads of the one-pass kernel
+template <bool CPX> struct G1 { static constexpr int NBW = CPX ? 4 : 5, NMAX = CPX ? 224 : 256; };
+
+template <class IN, class E, bool TSIDE, bool BF>
+__global__ __launch_bounds__(kG1T) void gram_big1_kernel(const IN* __restrict__ vol, GramArgs a) {
+  using R = typename real_of<E>::type;
+  static_assert(std::is_same<R, float>::value, "one-pass Gram: FP32 data");
+  constexpr bool CPX = is_cpx<E>::value;
+  constexpr int NPL = CPX ? 2 : 1;
+  constexpr int NBW = G1<CPX>::NBW, NMAX = G1<CPX>::NMAX, NW = kG1T / kWave;
+  constexpr int LW = NMAX + 1;
+  const Geom& g = a.g;
+  const int n = a.n, M = g.M, T = g.pt;
+  const int K = TSIDE ? M : T;
+  const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
+  const int lhi = lane >> 5, llo = lane & 31;
+  const int nb = (n + 31) / 32, npad = 32 * nb, NB = nb * (nb + 1) / 2;
+  extern __shared__ __align__(16) char smem[];
+  char* cur = smem;
+  IN* mean = lds_carve<IN>(cur, T);
+  int64_t* roff = lds_carve<int64_t>(cur, M);
+  E* S = BF ? nullptr : lds_carve<E>(cur, (size_t)kGBK * LW);  // S[k][i]
+  __bf16* Pl = BF ? lds_carve<__bf16>(cur, (size_t)NPL * 2 * NMAX * kGBW) : nullptr;
+  auto plane = [&](int ri, int hl) { return Pl + (size_t)(ri * 2 + hl) * NMAX * kGBW; };
+  // this wave's blocks (bi >= bj), fixed for the kernel
+  int bi[NBW], bj[NBW];
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) {
+    const int blk = wv + NW * b;
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= blk) ++r;
+    bi[b] = r;
+    bj[b] = blk - r * (r + 1) / 2;
+  }
+  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+    int64_t sp;
+    int t0;
+    patch_origin(g, a.idx[p], sp, t0);
+    const IN* X = vol + sp + t0;
+    for (int r = tid; r < M; r += kG1T) roff[r] = row_offset(g, r);
+    lds_barrier();
+    for (int t = tid; t < T; t += kG1T)
+      mean[t] = div_n(seq_add(X[roff[0] + t], X + t, roff + 1, M - 1), M);
+    floatx16 cre[NBW], cim[NBW];
+#pragma unroll
+    for (int b = 0; b < NBW; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) cre[b][q] = cim[b][q] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += kGBK) {
+      const int kc = min(kGBK, K - k0);
+      lds_barrier();  // means ready / previous chunk consumed
+      for (int q = tid; q < kGBK * npad; q += kG1T) {
+        int i, k;
+        if (TSIDE) {  // consecutive threads: consecutive frames of one row
+          k = q / npad;
+          i = q % npad;
+        } else {
+          i = q / kGBK;
+          k = q % kGBK;
+        }
+        E v = E(R(0));
+        if (k < kc && i < n) {
+          const int rr = TSIDE ? k0 + k : i, tt = TSIDE ? i : k0 + k;
+          v = cvt<E>(X[roff[rr] + tt] - mean[tt]);
+        }
+        if constexpr (BF) {
+          const float comp[2] = {(float)re_(v), (float)im_(v)};
+#pragma unroll
+          for (int ri = 0; ri < NPL; ++ri) {
+            const __bf16 hi = (__bf16)comp[ri];
+            plane(ri, 0)[i * kGBW + k] = hi;
+            plane(ri, 1)[i * kGBW + k] = (__bf16)(comp[ri] - (float)hi);
+          }
+        } else {
+          S[k * LW + i] = v;
+        }
+      }
+      lds_barrier();
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) {
+        if (wv + NW * b >= NB) continue;  // wave-uniform
+        const int ra = 32 * bi[b] + llo, rb = 32 * bj[b] + llo;
+        if constexpr (BF) {
+          auto ld = [&](int ri, int hl, int row, int k) {
+            return *reinterpret_cast<const bf16x8*>(plane(ri, hl) + row * kGBW + k + 8 * lhi);
+          };
+          auto prod3 = [&](floatx16 c, bf16x8 xh, bf16x8 xl, bf16x8 yh, bf16x8 yl) {
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, c, 0, 0, 0);
+            return __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, c, 0, 0, 0);
+          };
+#pragma unroll
+          for (int k = 0; k < kGBK; k += 16) {
+            const bf16x8 arh = ld(0, 0, ra, k), arl = ld(0, 1, ra, k);
+            const bf16x8 brh = ld(0, 0, rb, k), brl = ld(0, 1, rb, k);
+            cre[b] = prod3(cre[b], arh, arl, brh, brl);
+            if constexpr (CPX) {
+              const bf16x8 aih = ld(1, 0, ra, k), ail = ld(1, 1, ra, k);
+              const bf16x8 bih = ld(1, 0, rb, k), bil = ld(1, 1, rb, k);
+              cre[b] = prod3(cre[b], aih, ail, bih, bil);
+              if (TSIDE) {
+                cim[b] = prod3(cim[b], arh, arl, bih, bil);
+                cim[b] = prod3(cim[b], neg_bf16x8(aih), neg_bf16x8(ail), brh, brl);
+              } else {
+                cim[b] = prod3(cim[b], aih, ail, brh, brl);
+                cim[b] = prod3(cim[b], neg_bf16x8(arh), neg_bf16x8(arl), bih, bil);
+              }
+            }
+          }
+        } else {
+          for (int k = 0; k < kc; k += 2) {  // rows >= kc are zero
+            const E xa = S[(k + lhi) * LW + ra];
+            const E xb = S[(k + lhi) * LW + rb];
+            if constexpr (CPX) {
+              const float ar = re_(xa), ai = im_(xa), br = re_(xb), bim = im_(xb);
+              cre[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cre[b], 0, 0, 0);
+              cre[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bim, cre[b], 0, 0, 0);
+              if (TSIDE) {
+                cim[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bim, cim[b], 0, 0, 0);
+                cim[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, br, cim[b], 0, 0, 0);
+              } else {
+                cim[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, cim[b], 0, 0, 0);
+                cim[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(-ar, bim, cim[b], 0, 0, 0);
+              }
+            } else {
+              cre[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(re_(xa), re_(xb), cre[b], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    // blocks -> G (lower blocks and their mirrors)
+    E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) {
+      if (wv + NW * b >= NB) continue;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int r = 32 * bi[b] + 8 * (q >> 2) + 4 * lhi + (q & 3);
+        const int c = 32 * bj[b] + llo;
+        if (r < n && c < n) {
+          E x = mk<E>(cre[b][q], cim[b][q]);
+          if (a.scale) x = div_n(x, M - 1);
+          G[(size_t)r * n + c] = x;
+          if (bi[b] != bj[b]) G[(size_t)c * n + r] = conj_(x);
+        }
+      }
+    }
+  }
+}
+
+template <class IN, class E, bool BF>
+size_t gram_big1_lds_bytes(int M, int T) {
+  constexpr bool CPX = is_cpx<E>::value;
+  constexpr int NMAX = G1<CPX>::NMAX;
+  auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
+  const size_t st = BF ? al((size_t)(CPX ? 2 : 1) * 2 * NMAX * kGBW * 2)
+                       : al((size_t)kGBK * (NMAX + 1) * sizeof(E));
+  return al((size_t)T * sizeof(IN)) + al((size_t)M * sizeof(int64_t)) + st;
 }
 
 template <class IN, class E, bool BF = false>
@@ -2900,6 +3068,21 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
   const size_t lds = bf ? gram_big_lds_bytes<IN, E, true>(pl.g.M, pl.g.pt)
                         : gram_big_lds_bytes<IN, E>(pl.g.M, pl.g.pt);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
+  if constexpr (F32) {
+    constexpr bool CPX = is_cpx<E>::value;
+    if (pl.n <= G1<CPX>::NMAX) {
+      const size_t lds1 = bf ? gram_big1_lds_bytes<IN, E, true>(pl.g.M, pl.g.pt)
+                             : gram_big1_lds_bytes<IN, E, false>(pl.g.M, pl.g.pt);
+      if (lds1 <= 160 * 1024) {
+        auto k1 = bf ? gram_big1_kernel<IN, E, TSIDE, true> : gram_big1_kernel<IN, E, TSIDE, false>;
+        HIP_TRY(hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+        const int per_cu = pick_per_cu(pl.per_cu[0], 2, lds1);
+        k1<<<grid_for(ga.np, per_cu), kG1T, lds1, st>>>(reinterpret_cast<const IN*>(vol), ga);
+        HIP_TRY(hipGetLastError());
+        return PDAMD_OK;
+      }
+    }
+  }
   auto kern = bf ? gram_big_kernel<IN, E, TSIDE, F32> : gram_big_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = pick_per_cu(pl.per_cu[0], 4, lds);
