@@ -61,9 +61,9 @@ enum { PDAMD_MASK_NONE = 0, PDAMD_MASK_SPATIAL = 1, PDAMD_MASK_FULL = 2 };
 /* eigen-stage precision */
 /* PDAMD_PREC_BF16X3: FP32 eigen stage with the Gram from split-BF16
    three-product MFMA with FP32 accumulation (complex: four real products)
-   for FP32 real data with fewer voxels than frames and Gram size 65..128,
-   and for FP32 real / complex64 data with Gram size 129..512; other shapes
-   use the FP32 Gram.                                                        */
+   for FP32 real data with Gram size 65..128 (either side) and for FP32
+   real / complex64 data with Gram size 129..512; other shapes use the FP32
+   Gram.                                                                     */
 enum { PDAMD_PREC_AUTO = 0, PDAMD_PREC_FP32 = 1, PDAMD_PREC_FP64 = 2, PDAMD_PREC_BF16X3 = 3 };
 
 /* error codes */

@@ -404,10 +404,10 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   // FP32 real Grams of size 65..128 use MFMA 32x32x2: wave w owns the 64x64
   // block (w >> 1, w & 1), i.e. 2x2 tiles of 32x32 (16 accumulators per lane each).
   constexpr bool MF = std::is_same<E, float>::value && TILE == 8;
-  // Split-BF16 (M-side): each centred value x = hi + lo with hi = bf16(x),
+  // Split-BF16 (both sides): each centred value x = hi + lo with hi = bf16(x),
   // lo = bf16(x - hi); G += hi hi^T + hi lo^T + lo hi^T on MFMA 32x32x16 bf16
   // with FP32 accumulation (the lo lo^T term, ~2^-16 relative, is dropped).
-  constexpr bool BFM = BF && MF && !TSIDE && SC == 32;
+  constexpr bool BFM = BF && MF && SC == 32;  // M-side: planes [row][frame]; T-side: [frame][row]
   const int wv = tid / kWave, lane = tid % kWave;
   const int bi = wv >> 1, bj = wv & 1, lhi = lane >> 5, llo = lane & 31;
 
@@ -580,7 +580,13 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
           const int r = q / NB, t = q % NB;
           E c = E(R(0));
           if (r < rc && t < T) c = cvt<E>(Sraw[r * LDS_W + t] - mean[t]);
-          Sc[r * LDS_W + t] = c;
+          if constexpr (BFM) {  // Gram index t, reduction index r
+            const __bf16 hi = (__bf16)c;
+            Sh[t * kBfW + r] = hi;
+            Sl[t * kBfW + r] = (__bf16)(c - (float)hi);
+          } else {
+            Sc[r * LDS_W + t] = c;
+          }
         }
         lds_barrier();
         if constexpr (MF) {
@@ -649,7 +655,7 @@ size_t gram_lds_bytes(int n, int M) {
   size_t s = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? (size_t)M : (size_t)NB) * sizeof(int64_t));
-  if (BF && std::is_same<E, float>::value && TILE == 8 && !TSIDE && SC == 32)
+  if (BF && std::is_same<E, float>::value && TILE == 8 && SC == 32)
     s += 2 * al((size_t)NB * kBfW * 2);
   return s;
 }
@@ -2902,7 +2908,7 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
     // complex double Gram of n > 64 does not fit LDS with this kernel
   }
   pl->etype = (cplx ? 2 : 0) + (e64 ? 1 : 0);
-  // split-BF16: FP32 real M-side Grams of 65..128 (gram_kernel) and every
+  // split-BF16: FP32 real Grams of 65..128 (gram_kernel) and every
   // FP32 real / complex Gram above 128 (gram_big_kernel)
   pl->bf16x3 = P->precision == PDAMD_PREC_BF16X3 && !e64 && (!cplx || n > 128);
   pl->rbytes = e64 ? 8 : 4;
@@ -3120,7 +3126,7 @@ int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t s
     case 2: return launch_gram<IN, E, TSIDE, 2>(pl, vol, ga, st);
     case 4: return launch_gram<IN, E, TSIDE, 4>(pl, vol, ga, st);
     default:
-      if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value && !TSIDE)
+      if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value)
         if (pl.bf16x3) return launch_gram<IN, E, TSIDE, 8, true>(pl, vol, ga, st);
       return launch_gram<IN, E, TSIDE, 8>(pl, vol, ga, st);
   }

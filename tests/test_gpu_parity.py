@@ -157,3 +157,25 @@ def test_cfg1_full_run_against_fixture():
     # whole-volume checksums of the reference output
     s = np.nansum(out.astype(np.float64))
     assert abs(s - float(z["out_nansum"])) <= 1e-6 * abs(float(z["out_nansum"]))
+
+
+def test_split_bf16_tside_gram_against_oracle():
+    """precision="bf16x3" on a T-side Gram (5^3 voxels > 100 frames, Gram 100):
+    split-BF16 three-product MFMA with FP32 accumulation, hi / lo planes
+    stored frame-major.  Bound 1e-3 against the oracle (the split drops the
+    lo x lo term, ~2^-16 relative)."""
+    from oracle import cpu_ref as O
+
+    pd = _pd()
+    rng = np.random.default_rng(12)
+    shape, T = (11, 10, 9), 100
+    sig = 500 + 20 * np.tensordot(rng.standard_normal((4,) + shape),
+                                  rng.standard_normal((4, T)), axes=([0], [0]))
+    data = (sig + 10 * rng.standard_normal(shape + (T,))).astype(np.float32)
+    out, w, _, _ = pd.mp_pca(data, 5, 4, precision="bf16x3")
+    ref32, _, _, _ = pd.mp_pca(data, 5, 4, precision="fp32")
+    want, want_w, _, _ = O.mp_pca(data, 5, 4)
+    e, e32 = rel_l2(out, want), rel_l2(out, ref32)
+    print(f"[gpu-bf16] T-side Gram 100: rel-L2 vs oracle {e:.3g}, vs fp32 path {e32:.3g}", flush=True)
+    assert np.array_equal(np.isnan(out), np.isnan(want))
+    assert e <= 1e-3 and e32 > 0.0  # the split path really ran (not bit-identical to FP32)
