@@ -183,3 +183,25 @@ def test_adaptive_errors_match_reference():
         pd.adaptive_thresholding(data, 3, 2, method="SURE", gamma0=None)
     with pytest.raises(TypeError):  # AdaptiveDenoiser.denoise: noise_std None is squared
         pd.AdaptiveDenoiser(3, 2, method="SURE").denoise(data, gamma0=[1.0, 2.0])
+
+
+def test_adaptive_large_gram_against_oracle():
+    """Gram sizes above 128 (the large-Gram kernels with the adaptive eigen
+    rule): 7^3-voxel patches of 200 frames (T-side, n = 200) and 6^3 x 260
+    (M-side, n = 216), float32, SURE and GSURE."""
+    import warnings
+
+    import patch_denoise as pd
+    from oracle import cpu_ref as O
+
+    for shape, p, method in (((8, 8, 8, 200), 7, "SURE"), ((7, 7, 7, 260), 6, "GSURE")):
+        _, data = _volume(13, shape=shape)
+        kw = dict(method=method, gamma0=np.linspace(1, 4, 5), noise_std=2.0)
+        out, w, _, _ = pd.adaptive_thresholding(data, p, p - 1, **kw)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            want, want_w, _, _ = O.adaptive_thresholding(data, p, p - 1, **kw)
+        e = rel_l2(out, want)
+        print(f"[adaptive] large Gram {shape} p {p} {method}: rel-L2 {e:.3g}", flush=True)
+        assert np.array_equal(np.isnan(out), np.isnan(want))
+        assert e <= 1e-5 and rel_l2(w, want_w) <= 1e-6
