@@ -180,7 +180,7 @@ def bench_differentiable(steps):
                                    if cnt[i]})
 
 
-def bench_batched(steps):
+def bench_batched(steps, autotune=True):
     """The eight-volume mixed workload (seed 4, all eight volumes, Gram sizes
     27 to 200): one pdamd_denoise call per volume vs
     one pdamd_denoise_batch call (eager, and replayed from its captured
@@ -213,20 +213,22 @@ def bench_batched(steps):
     t_eager = timed(eager.run)
     t_graph = timed(graph.run)
     del graph
-    t0 = time.perf_counter()
-    tuned = BatchPlan(calls, use_graph=True, autotune=True)
-    tune_s = time.perf_counter() - t0
-    t_tuned = timed(tuned.run)
+    tuned_res = {}
+    if autotune:
+        t0 = time.perf_counter()
+        tuned = BatchPlan(calls, use_graph=True, autotune=True)
+        tune_s = time.perf_counter() - t0
+        t_tuned = timed(tuned.run)
+        tuned_res = dict(autotuned_graph_ms=t_tuned * 1e3, autotune_s=tune_s,
+                         autotuned_launch_per_cu=[list(t) for t in tuned.tuned],
+                         speedup_autotuned_vs_sequential=t_seq / t_tuned)
     return dict(workload="batched_8vol_seed4", volumes=[
         dict(method=c[0], shape=list(c[1][0].shape), patch=c[1][1]) for c in calls],
         skipped=[dict(index=i, reason=r) for i, r in skipped], voxel_frames=vf,
         patches=eager.batch.selected, steps=steps, datagen_s=gen_s,
         sequential_ms=t_seq * 1e3, batch_ms=t_eager * 1e3, batch_graph_ms=t_graph * 1e3,
         sequential_voxel_frames_s=vf / t_seq, batch_graph_voxel_frames_s=vf / t_graph,
-        speedup_graph_vs_sequential=t_seq / t_graph,
-        autotuned_graph_ms=t_tuned * 1e3, autotune_s=tune_s,
-        autotuned_launch_per_cu=[list(t) for t in tuned.tuned],
-        speedup_autotuned_vs_sequential=t_seq / t_tuned)
+        speedup_graph_vs_sequential=t_seq / t_graph, **tuned_res)
 
 
 def bench_large(steps):
@@ -329,6 +331,8 @@ def main():
                     help="steps of the batched multi-volume measurement; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiable (fwd / fwd+bwd) measurement; 0 skips it")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the batched leg's autotuned measurement (for kernel traces)")
     ap.add_argument("--large-steps", type=int, default=2,
                     help="steps of the Gram-size-200 measurements; 0 skips them")
     ap.add_argument("--only", choices=["all", "headline", "differentiable", "batched", "noise",
@@ -353,7 +357,8 @@ def main():
         print(json.dumps({"differentiable": bench_differentiable(max(args.diff_steps, 1))}))
         return
     if args.only == "batched":
-        print(json.dumps({"batched": bench_batched(max(args.batch_steps, 1))}))
+        print(json.dumps({"batched": bench_batched(max(args.batch_steps, 1),
+                                                   autotune=not args.no_autotune)}))
         return
     if args.only == "noise":
         print(json.dumps({"noise_map": bench_noise_map(5)}))
@@ -451,7 +456,8 @@ def main():
 
     full = args.only == "all"
     diff = bench_differentiable(args.diff_steps) if full and args.diff_steps > 0 else None
-    batched = bench_batched(args.batch_steps) if full and args.batch_steps > 0 else None
+    batched = (bench_batched(args.batch_steps, autotune=not args.no_autotune)
+               if full and args.batch_steps > 0 else None)
     noise_map = bench_noise_map(5) if full else None
     large = bench_large(args.large_steps) if full and args.large_steps > 0 else None
 
