@@ -395,7 +395,10 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   using R = typename real_of<E>::type;
   constexpr int NB = 16 * TILE;  // padded Gram size
   constexpr bool SAME = std::is_same<IN, E>::value;
-  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
+  // M-side FP32 MFMA path (no split): 64 frames per staged chunk, so 64
+  // threads compute the per-frame means (sequential over rows) at a time
+  constexpr int SC = TSIDE ? Stage<E>::RC
+                           : ((std::is_same<E, float>::value && TILE == 8 && !BF) ? 64 : Stage<E>::TC);
   constexpr int LDS_W = NB + 1;  // staging row length (padded)
   const Geom& g = a.g;
   const int n = a.n;
@@ -648,7 +651,8 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 size_t gram_lds_bytes(int n, int M) {
   constexpr int NB = 16 * TILE;
-  constexpr int SC = TSIDE ? Stage<E>::RC : Stage<E>::TC;
+  constexpr int SC = TSIDE ? Stage<E>::RC
+                           : ((std::is_same<E, float>::value && TILE == 8 && !BF) ? 64 : Stage<E>::TC);
   constexpr int LDS_W = NB + 1;
   constexpr bool SAME = std::is_same<IN, E>::value;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
