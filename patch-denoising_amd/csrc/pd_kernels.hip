@@ -363,6 +363,7 @@ struct GramArgs {
   void* H;             // np x n x n   Householder rows
   void* DE;            // np x 3n      d | e | e^2 (real type)
   void* TAU;           // np x n
+  void* MEAN;          // np x pt per-frame patch means (input type), reused by the synthesis
 };
 
 // Staging geometry: M-side stages kTC frames x (all rows); T-side stages kRC
@@ -573,6 +574,7 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
         lds_barrier();
       }
       if (tid < NB) mean[tid] = tid < T ? div_n(s, M) : IN(0);
+      if (tid < T) reinterpret_cast<IN*>(a.MEAN)[(size_t)p * T + tid] = mean[tid];
       load(0);
       for (int r0 = 0; r0 < M; r0 += SC) {
         const int rc = min(SC, M - r0);
@@ -782,8 +784,10 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
     for (int r = tid; r < M; r += kNT) roff[r] = row_offset(g, r);
     lds_barrier();
     // per-frame mean over the rows, added in row order (numpy's axis-0 reduction)
-    for (int t = tid; t < T; t += kNT)
+    for (int t = tid; t < T; t += kNT) {
       mean[t] = div_n(seq_add(X[roff[0] + t], X + t, roff + 1, M - 1), M);
+      if (TSIDE) reinterpret_cast<IN*>(a.MEAN)[(size_t)p * T + t] = mean[t];
+    }
     lds_barrier();
     E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     // element (i, k) of the n x K operand, centred: T-side Xc[k][i], M-side Xc[i][k]
@@ -1005,8 +1009,10 @@ __global__ __launch_bounds__(kG1T) void gram_big1_kernel(const IN* __restrict__ 
     const IN* X = vol + sp + t0;
     for (int r = tid; r < M; r += kG1T) roff[r] = row_offset(g, r);
     lds_barrier();
-    for (int t = tid; t < T; t += kG1T)
+    for (int t = tid; t < T; t += kG1T) {
       mean[t] = div_n(seq_add(X[roff[0] + t], X + t, roff + 1, M - 1), M);
+      if (TSIDE) reinterpret_cast<IN*>(a.MEAN)[(size_t)p * T + t] = mean[t];
+    }
     floatx16 cre[NBW], cim[NBW];
 #pragma unroll
     for (int b = 0; b < NBW; ++b)
@@ -2451,6 +2457,7 @@ struct SynthArgs {
   const double* WC;
   const int* NCOMP;
   const double* THETA;
+  const void* MEAN;  // np x pt per-frame patch means written by the Gram kernels
   void* out;
   double* w;  // w3d ? voxel weights : element weights (float64 accumulators)
 };
@@ -2502,6 +2509,7 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
     const double theta = a.recomb == PDAMD_WEIGHTED ? a.THETA[p] : 1.0;
     const E* Yp = reinterpret_cast<const E*>(a.Y) + (size_t)p * n * a.kmax;
     const double* WCp = a.WC + (size_t)p * a.kmax;
+    const IN* MEANp = reinterpret_cast<const IN*>(a.MEAN) + (size_t)p * T;
     const bool center = a.recomb == PDAMD_CENTER;
     const int ctr_r = ((g.p0 / 2) * g.p1 + g.p1 / 2) * g.p2 + g.p2 / 2;
     const int ctr_t = T / 2;
@@ -2587,44 +2595,8 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
         lds_barrier();
       }
     } else {
-      // pass 1: per-frame means over all rows (sequential order); thread tid
-      // sums frames tid + kNT j
-      constexpr int NF = (SC == kSynSC) ? 1 : kNBig / kNT;  // frames per thread (T <= NF kNT)
-      IN sm[NF];
-#pragma unroll
-      for (int j = 0; j < NF; ++j) sm[j] = IN(0);
-      for (int r0 = 0; r0 < M; r0 += SC) {
-        const int rc = min(SC, M - r0);
-        for (int r = tid; r < rc; r += kNT) roff[r] = row_offset(g, r0 + r);
-        lds_barrier();
-        {
-          constexpr int NL = kSynNI;
-          IN buf[NL];
-#pragma unroll
-          for (int u = 0; u < NL; ++u) {
-            const int q = tid + u * kNT;
-            buf[u] = q < rc * T ? X[roff[q / T] + q % T] : IN(0);
-          }
-#pragma unroll
-          for (int u = 0; u < NL; ++u) {
-            const int q = tid + u * kNT;
-            if (q < rc * T) Sraw[(q / T) * ldw + q % T] = buf[u];
-          }
-        }
-        lds_barrier();
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-          const int t = tid + j * kNT;
-          if (t < T) {
-            if (r0 == 0) sm[j] = seq_add(Sraw[t], Sraw + ldw + t, rc - 1, ldw);
-            else sm[j] = seq_add(sm[j], Sraw + t, rc, ldw);
-          }
-        }
-        lds_barrier();
-      }
-#pragma unroll
-      for (int j = 0; j < NF; ++j)
-        if (tid + j * kNT < T) mean[tid + j * kNT] = div_n(sm[j], M);
+      // per-frame means over all rows: the Gram kernel's (numpy order)
+      for (int t = tid; t < T; t += kNT) mean[t] = MEANp[t];
       lds_barrier();
       for (int r0 = 0; r0 < M; r0 += SC) {
         const int rc = min(SC, M - r0);
@@ -2835,6 +2807,7 @@ struct Plan {
   int64_t chunk;
   size_t ebytes, rbytes;
   // workspace offsets
+  size_t off_MEAN = 0;
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
       off_WC, off_NC, off_TH, off_w3, off_SCR, total;
   // backward only
@@ -2955,6 +2928,10 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   pl->off_DE = take(C * 3 * n * pl->rbytes);
   pl->off_SCR = take(C * 3 * n * (bwd ? n : pl->kb) * pl->rbytes);
   pl->off_TAU = take(C * n * pl->ebytes);
+  if (pl->tside) {  // per-frame means of T-side patches, from the Gram to the synthesis
+    const size_t inb = P->dtype == PDAMD_F32 ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
+    pl->off_MEAN = take(C * (size_t)pt * inb);
+  }
   pl->off_Y = take(C * n * pl->kmax * pl->ebytes);
   pl->off_Z = take(C * n * pl->kmax * pl->rbytes);
   pl->off_WC = take(C * pl->kmax * sizeof(double));
@@ -3247,6 +3224,7 @@ GramArgs gram_args(const pdamd_problem* P, const Plan& pl, char* ws, const int32
   ga.H = ws + pl.off_H;
   ga.DE = ws + pl.off_DE;
   ga.TAU = ws + pl.off_TAU;
+  ga.MEAN = ws + pl.off_MEAN;
   return ga;
 }
 
@@ -3290,6 +3268,7 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     sa.WC = ea.WC;
     sa.NCOMP = ea.NCOMP;
     sa.THETA = ea.THETA;
+    sa.MEAN = ga.MEAN;
     sa.out = P->output_dev;
     sa.w = wacc;
     tm.begin(PDAMD_STAGE_SYNTH);
