@@ -67,22 +67,23 @@ def _tuple_positive_int(string, is_parser=True):
     return tuple(_positive_int(s, is_parser) for s in string.split(sep.group(0)))
 
 
-class ToDict(argparse.Action):
-    """``key=value`` pairs -> dict; values that parse as float become floats."""
+def _key_value(spec):
+    """``"key=value"`` -> (key, value); the value becomes a float when it parses
+    as one.  Anything but exactly one ``=`` is rejected (cli.py:91-112)."""
+    key, sep, value = spec.partition("=")
+    if not sep or "=" in value:
+        raise ValueError("Extra arguments must be in the form key=value.")
+    try:
+        return key, float(value)
+    except ValueError:
+        return key, value
 
-    def __call__(self, parser, namespace, values, option_string=None):  # noqa: U100
-        d = {}
-        for spec in values:
-            try:
-                key, value = spec.split("=")
-            except ValueError:
-                raise ValueError("Extra arguments must be in the form key=value.") from None
-            try:
-                value = float(value)
-            except ValueError:
-                pass
-            d[key] = value
-        setattr(namespace, self.dest, d)
+
+class ToDict(argparse.Action):
+    """argparse action: a list of ``key=value`` strings -> dict (later keys win)."""
+
+    def __call__(self, parser, namespace, values, option_string=None):
+        setattr(namespace, self.dest, dict(_key_value(spec) for spec in values))
 
 
 def get_parser():

@@ -235,25 +235,36 @@ class BaseSpaceTimeDenoiser(abc.ABC):
         raise NotImplementedError
 
     def _get_patch_param(self, data_shape):
-        """Per-dimension patch shape and overlap (base.py:359-384)."""
-        pp = [None, None]
-        for i, attr in enumerate(["p_shape", "p_ovl"]):
-            p = getattr(self, attr)
-            if isinstance(p, list):
-                p = tuple(p)
-            elif isinstance(p, (int, np.integer)):
-                p = (p,) * (len(data_shape) - 1)
-            if len(p) == len(data_shape) - 1:
-                p = (*p, data_shape[-1])
-            pp[i] = p
-        if np.prod(pp[0][:-1]) < data_shape[-1]:
+        """Per-dimension patch shape and overlap (base.py:359-384).
+
+        Both specs go through the same expansion; the ill-conditioning
+        warning (same text as the reference) fires when the patch has fewer
+        voxels than the volume has frames.
+        """
+        shape = _expand_patch_spec(self.p_shape, data_shape)
+        overlap = _expand_patch_spec(self.p_ovl, data_shape)
+        if np.prod(shape[:-1]) < data_shape[-1]:
             logging.warning(
-                f"the number of voxel in patch ({np.prod(pp[0])}) is smaller than the"
+                f"the number of voxel in patch ({np.prod(shape)}) is smaller than the"
                 f" last dimension ({data_shape[-1]}), this makes an ill-conditioned"
                 "matrix for SVD.",
                 stacklevel=2,
             )
-        return tuple(pp)
+        return shape, overlap
+
+
+def _expand_patch_spec(spec, data_shape):
+    """Patch spec as a per-axis tuple (base.py:365-373).
+
+    An integer applies to every spatial axis; a list becomes a tuple.  A spec
+    naming only the spatial axes is completed with the full time length (one
+    temporal patch, time step 0); any other length is kept as given.
+    """
+    n_space = len(data_shape) - 1
+    dims = (spec,) * n_space if isinstance(spec, (int, np.integer)) else spec
+    if isinstance(dims, list):
+        dims = tuple(dims)
+    return (*dims, data_shape[-1]) if len(dims) == n_space else dims
 
 
 def _progress(progbar, n):
