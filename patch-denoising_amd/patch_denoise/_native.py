@@ -309,10 +309,11 @@ class BatchRun:
     reused by every :meth:`run`, so repeated runs can replay the captured graph.
     """
 
-    def __init__(self, items, use_graph=True):
+    def __init__(self, items, use_graph=True, want_patch_info=False):
         torch = _torch()
         self.torch = torch
-        self.runs = [DeviceRun(prob, data, mask, var_map, workspace=False)
+        self.runs = [DeviceRun(prob, data, mask, var_map, want_patch_info=want_patch_info,
+                               workspace=False)
                      for prob, data, mask, var_map in items]
         n = len(self.runs)
         self.probs = (Problem * n)()
@@ -337,6 +338,15 @@ class BatchRun:
 
     def results(self, shapes):
         return [r.result(s) for r, s in zip(self.runs, shapes)]
+
+    def patch_info(self):
+        """Per-volume selected patch indices, ranks, noise and eigenvalues
+        (only when constructed with ``want_patch_info``)."""
+        out = []
+        for r, p in zip(self.runs, self.probs):
+            r.prob.selected = p.selected
+            out.append(r.patch_info())
+        return out
 
 
 def estimate_noise(data, block_size):

@@ -49,7 +49,7 @@ class BatchPlan:
     ``use_graph``).  Inputs stay as given at construction.  ``autotune``
     chooses each volume's launch shapes first (``patch_denoise.autotune``)."""
 
-    def __init__(self, calls, use_graph=True, autotune=False):
+    def __init__(self, calls, use_graph=True, autotune=False, want_patch_info=False):
         self.jobs = _collect(list(calls))
         self.tuned = None
         if autotune:  # launch shapes per volume, before any graph is captured
@@ -57,11 +57,16 @@ class BatchPlan:
 
             self.tuned = [_tune.tune(j.prob, j.data, j.mask_arr, j.var_map)[0] for j in self.jobs]
         self.batch = N.BatchRun([(j.prob, j.data, j.mask_arr, j.var_map) for j in self.jobs],
-                                use_graph=use_graph)
+                                use_graph=use_graph, want_patch_info=want_patch_info)
 
     def run(self):
         self.batch.run()
         return self
+
+    def patch_info(self):
+        """Per volume: selected patch indices, retained ranks, noise and top
+        eigenvalues (``want_patch_info=True`` at construction)."""
+        return self.batch.patch_info()
 
     def results(self):
         res = self.batch.results([j.data.shape for j in self.jobs])

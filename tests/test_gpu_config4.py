@@ -1,0 +1,131 @@
+"""Batched eight-volume workload (seed 4, full scale) against the CPU oracle.
+
+The eight float32 volumes of ``workloads.config4()`` (mixed shapes, T in
+{80..400}, patch 3 / 5 / 7, per-volume ellipsoid masks; MP-PCA, Hybrid-PCA
+with a per-voxel sigma map, optimal thresholding and raw SVT) go through ONE
+batched device call replayed from its hipGraph.  For every volume:
+
+* two 4x4x4-voxel windows, one on the mask boundary and one inside, are
+  recomputed by the oracle (every selected patch covering the window, the
+  reference's weighted recombination, out / w, mask zeroing) and compared
+  with the GPU output (rel-L2 <= 1e-5) and weights (<= 1e-6);
+* the retained rank of a strided sample of selected patches equals the
+  oracle's, except where the oracle's own decision margin shows a tie at the
+  cutoff (``tests/oracle_pool.py``).
+
+References: space_time/base.py:177-270 (driver, recombination),
+lowrank.py:42-63 (MP-PCA), :104-119 (Hybrid), :170-184 (raw SVT),
+:379-407 (optimal).
+"""
+
+import numpy as np
+import pytest
+
+from tests import oracle_pool as OP
+
+pytestmark = pytest.mark.gpu
+
+TIE = 1e-5  # decision margin (relative to the largest eigen / singular value) of a tie
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def _oracle_rule(vol, T, p):
+    from oracle import cpu_ref as O
+
+    m = vol["method"]
+    if m == "mppca":
+        return "mppca", {"ts": 1.0}
+    if m == "hybrid":
+        return "hybrid", {"var": np.asarray(vol["sigma_map"], np.float64) ** 2}
+    if m == "raw":
+        return "raw", {"tau": float(vol["tau"]) * 1.0}
+    if m == "optimal":
+        return "optimal", {"loss": "fro", "mpm": O.mp_median(T / p**3), "var": None}
+    raise ValueError(m)
+
+
+def _selected_by_box_sums(mask, p, thr=50):
+    """Patches kept by the selection rule (base.py:201-227) for a spatial mask
+    and a p^3 patch at stride 1, from cumulative box sums."""
+    c = np.pad(mask.astype(np.int64), ((1, 0), (1, 0), (1, 0))).cumsum(0).cumsum(1).cumsum(2)
+    a, b = slice(p, None), slice(None, -p)
+    s = (c[a, a, a] - c[b, a, a] - c[a, b, a] - c[a, a, b]
+         + c[b, b, a] + c[b, a, b] + c[a, b, b] - c[b, b, b])
+    return np.flatnonzero((100 * s / p**3 > thr).ravel())
+
+
+def _windows(shape, mask):
+    """An interior window at the centre and a boundary window straddling the
+    ellipsoid's surface along the first axis."""
+    c = [int((s - 1) / 2) for s in shape]
+    inner = tuple(ci - 2 for ci in c)
+    x_edge = int(round((shape[0] - 1) / 2 + 0.45 * shape[0]))
+    edge = (min(x_edge - 2, shape[0] - 4), c[1] - 2, c[2] - 2)
+    win = mask[edge[0]:edge[0] + 4, edge[1]:edge[1] + 4, edge[2]:edge[2] + 4]
+    assert win.any() and not win.all(), "boundary window must straddle the mask"
+    return [("boundary", edge), ("inside", inner)]
+
+
+def test_config4_batched_against_oracle():
+    from patch_denoise import workloads
+    from patch_denoise.batch import BatchPlan
+
+    vols = workloads.config4()
+    calls, skipped = workloads.config4_calls(vols)
+    assert len(calls) == 8 and not skipped
+    print("[config4] batched run (graph, replayed)", flush=True)
+    plan = BatchPlan(calls, use_graph=True, want_patch_info=True)
+    plan.run()
+    first = plan.results()
+    res = plan.run().results()  # replay of the captured graph
+    infos = plan.patch_info()
+    pool = OP._pool()
+    try:
+        for k, (vol, (name, args, kw), (out, w, _, _), info) in enumerate(
+                zip(vols, calls, res, infos)):
+            data, p = vol["data"], vol["patch"]
+            T = data.shape[-1]
+            ps, po = (p, p, p, T), (p - 1, p - 1, p - 1, T)
+            e_rep = rel_l2(np.nan_to_num(out), np.nan_to_num(first[k][0]))
+            assert e_rep < 1e-6, (k, e_rep)
+            method, params = _oracle_rule(vol, T, p)
+            sel = info["index"].astype(np.int64)
+            gpu_rank = dict(zip(sel.tolist(), info["rank"].tolist()))
+            grid = tuple((s - p) // 1 + 1 for s in data.shape[:3]) + (1,)
+            mask = vol["mask"]
+            assert np.array_equal(sel, _selected_by_box_sums(mask, p)), k
+            with OP.SharedVolume(data) as sv:
+                # strided rank sample
+                sample = sel[:: max(1, len(sel) // 400)]
+                r_or, margin = OP.oracle_ranks(sv, ps, po, method, params, sample, pool)
+                r_gpu = np.array([gpu_rank[int(L)] for L in sample])
+                bad = np.nonzero(r_or != r_gpu)[0]
+                print(f"[config4] volume {k} {name} {data.shape} p={p}: {len(sel)} patches, "
+                      f"rank mismatches {len(bad)} / {len(sample)} "
+                      f"(margins {[f'{margin[b]:.1e}' for b in bad]})", flush=True)
+                assert all(margin[b] < TIE for b in bad), "rank mismatch away from the cutoff"
+                for tag, lo in _windows(data.shape[:3], mask):
+                    hi = tuple(v + 4 for v in lo)
+                    ref, ref_w, wr = OP.oracle_window(sv, ps, po, method, params, sel, grid,
+                                                      lo, hi, pool)
+                    m = mask[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+                    ref = np.where(m[..., None], ref, 0.0)
+                    win = out[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+                    win_w = w[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+                    assert np.array_equal(np.isnan(win), np.isnan(ref))
+                    e = rel_l2(np.nan_to_num(win), np.nan_to_num(ref))
+                    ew = rel_l2(win_w, ref_w)
+                    diff = [L for L, r in wr.items() if gpu_rank[L] != r]
+                    print(f"[config4]   {tag} window {lo}: {len(wr)} patches, rel-L2 out "
+                          f"{e:.2e} weights {ew:.2e}, rank differences {len(diff)}", flush=True)
+                    if diff:  # only ties may differ; they move the window by O(1/r)
+                        _, mg = OP.oracle_ranks(sv, ps, po, method, params, diff, pool)
+                        assert np.all(mg < TIE), mg
+                        assert e < 1e-3
+                    else:
+                        assert e < 1e-5 and ew < 1e-6
+    finally:
+        pool.shutdown()

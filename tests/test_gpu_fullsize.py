@@ -4,7 +4,9 @@ The GPU denoises the whole volume; the CPU oracle then recomputes, for a
 small window of voxels, every selected patch covering the window and the
 reference's recombination (weighted: theta = 1/(2+r), out / w) on those
 voxels only.  The window output and weights must match, and the retained rank
-of a strided sample of patches must equal the oracle's.
+of every selected patch must equal the oracle's (computed by a CPU process
+pool, tests/oracle_pool.py) except at ties: where the oracle's own decision
+margin is within float32 rounding of the patch's largest eigenvalue.
 """
 
 import numpy as np
@@ -71,15 +73,18 @@ def test_mppca_80x80x48x300_window_and_ranks():
     out, w, info = _run_gpu(den, data, mask)
     sel = info["index"].astype(np.int64)
     assert len(sel) == 107680
-    # ranks on a strided sample, oracle per patch
-    src = O.StridedPatches(data, ps, po)
-    sample = np.arange(0, len(sel), 211)
-    mism = 0
-    for j in sample:
-        _, r, _ = O.mppca_patch(src.get(sel[j]).reshape(125, 300))
-        mism += int(r != info["rank"][j])
-    print(f"[fullsize] rank mismatches {mism} / {len(sample)}", flush=True)
-    assert mism <= max(1, len(sample) // 200)  # FP32 near-ties only
+    # every selected patch's rank against the oracle (CPU process pool); a
+    # mismatch is allowed only at a tie: the oracle's own decision margin
+    # within float32 rounding of the largest eigenvalue
+    from tests import oracle_pool as OP
+
+    with OP.SharedVolume(data) as sv:
+        r_or, margin = OP.oracle_ranks(sv, ps, po, "mppca", {"ts": 1.0}, sel)
+    bad = np.nonzero(r_or != info["rank"])[0]
+    print(f"[fullsize] rank mismatches {len(bad)} / {len(sel)} (all patches), margins "
+          f"{[f'{margin[b]:.1e}' for b in bad[:20]]}", flush=True)
+    assert np.all(margin[bad] < 1e-5), "rank mismatch away from the cutoff"
+    assert len(bad) <= 20
     # a 4x4x4-voxel window at the mask boundary and one inside
     for lo in ((38, 38, 22), (6, 38, 22)):
         hi = tuple(v + 4 for v in lo)
@@ -108,14 +113,15 @@ def test_optimal_fro_64x64x32x120_window():
     print("[fullsize] optimal-fro 64x64x32x120 gpu run", flush=True)
     out, w, info = _run_gpu(den, data, None)
     sel = info["index"].astype(np.int64)
-    src = O.StridedPatches(data, ps, po)
-    sample = np.arange(0, len(sel), 397)
-    mism = 0
-    for j in sample:
-        _, r, _ = O.optimal_patch(src.get(sel[j]).reshape(125, 120), O.shrink_fro, mpm)
-        mism += int(r != info["rank"][j])
-    print(f"[fullsize] rank mismatches {mism} / {len(sample)}", flush=True)
-    assert mism <= max(1, len(sample) // 100)
+    from tests import oracle_pool as OP
+
+    with OP.SharedVolume(data) as sv:
+        r_or, margin = OP.oracle_ranks(sv, ps, po, "optimal",
+                                       {"loss": "fro", "mpm": mpm, "var": None}, sel)
+    bad = np.nonzero(r_or != info["rank"])[0]
+    print(f"[fullsize] rank mismatches {len(bad)} / {len(sel)} (all patches), margins "
+          f"{[f'{margin[b]:.1e}' for b in bad[:20]]}", flush=True)
+    assert np.all(margin[bad] < 1e-5), "rank mismatch away from the cutoff"
     lo = (30, 30, 14)
     hi = tuple(v + 3 for v in lo)
     ref, ref_w, _ = _window_reference(
