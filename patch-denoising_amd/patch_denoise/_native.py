@@ -98,6 +98,12 @@ def load_library():
     global _lib
     if _lib is not None:
         return _lib
+    # torch's own HIP runtime libraries must be in the process before this
+    # library pulls in the system ones: loaded the other way round, this
+    # library's HIP calls later report that no device exists.  Importing
+    # torch loads them without touching the GPU.
+    import torch  # noqa: F401
+
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"native library {LIB_PATH} is not built; run "
