@@ -226,6 +226,20 @@ int pdamd_estimate_noise(const void* in_dev, int dtype, const int64_t* shape4, i
                          int block_size, double* out_dev, void* workspace_dev,
                          size_t workspace_bytes, void* stream);
 
+/* pdamd_denoise without any device allocation, for callers that cannot call
+   hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+   that forbid allocation).  No counterpart in the reference.  The workspace
+   is a fixed device array of pdamd_noalloc_scratch_bytes() bytes (256 MiB)
+   held by every GPU the library runs on until the process ends; patches are
+   processed in chunks sized to fit it.  Problems whose per-volume arrays
+   plus one patch exceed it fail with PDAMD_EUNSUPPORTED.  The caller still
+   passes the input, output and weight buffers (and the optional per-patch
+   arrays) in `prob`; stage timing is not available here.  Calls on one GPU
+   share its array: calls that overlap on one GPU race.  Results equal
+   pdamd_denoise's up to the summation order of overlapping patches.      */
+int pdamd_noalloc_scratch_bytes(size_t* bytes);
+int pdamd_denoise_noalloc(pdamd_problem* prob, void* stream);
+
 const char* pdamd_last_error(void);
 const char* pdamd_version(void);
 

@@ -2816,7 +2816,8 @@ struct Plan {
 };
 
 // bwd: layout for pdamd_optimal_backward (all eigenpairs, adjoint buffers).
-int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
+// chunk_cap > 0 limits the patches per chunk (workspace of a fixed size).
+int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_cap = 0) {
   if (!P) return fail(PDAMD_EINVAL, "null problem");
   if (P->dtype < 0 || P->dtype > 3) return fail(PDAMD_EINVAL, "bad dtype %d", P->dtype);
   for (int i = 0; i < 4; ++i) {
@@ -2910,6 +2911,7 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false) {
   int64_t chunk = (int64_t)((bwd ? (6ull << 30) : (4ull << 30)) / per);
   chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
   pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
+  if (chunk_cap > 0) pl->chunk = std::min<int64_t>(pl->chunk, chunk_cap);
   // workspace layout
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -3452,6 +3454,73 @@ std::vector<GraphEntry>& graph_cache() {
 
 namespace pdamd_detail {
 void set_error(const char* msg) { g_err = msg; }
+
+// The body of pdamd_denoise for an already-made plan.
+int denoise_planned(pdamd_problem* P, const void* plan, void* workspace_dev,
+                    size_t workspace_bytes, void* stream) {
+  const Plan& pl = *reinterpret_cast<const Plan*>(plan);
+  if (workspace_bytes < pl.total)
+    return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, pl.total);
+  if (!P->input_dev || !P->output_dev || !P->weights_dev)
+    return fail(PDAMD_EINVAL, "null data pointer");
+  if ((P->method == PDAMD_HYBRID || P->method == PDAMD_ADAPTIVE) && !P->var_map_dev)
+    return fail(PDAMD_EINVAL, "hybrid / adaptive rules need a variance map");
+  if (P->method == PDAMD_ADAPTIVE && P->adaptive_method == PDAMD_ADAPT_QUT && !P->qut_tau_dev)
+    return fail(PDAMD_EINVAL, "QUT needs the per-patch threshold array");
+  if (P->lam_dev && P->method != PDAMD_OPTIMAL)
+    return fail(PDAMD_EINVAL, "a threshold scale map is only used by the optimal rule");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* ws = reinterpret_cast<char*>(workspace_dev);
+  const size_t esz = (P->dtype == PDAMD_F32) ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
+  HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
+  if (P->recombination == PDAMD_CENTER)  // otherwise the normalisation writes every weight
+    HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
+  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0,
+                         (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st));
+  int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
+  int64_t count = 0;
+  StageTimer tm;
+  tm.on = P->stage_ms != nullptr || P->stage_launches != nullptr;
+  tm.st = st;
+  int rc = run_select(P, pl, ws, idx, &count, st, &tm);
+  if (rc) return rc;
+  P->selected = count;
+  if (P->patch_index_dev && count > 0)
+    HIP_TRY(hipMemcpyAsync(P->patch_index_dev, idx, count * sizeof(int32_t),
+                           hipMemcpyDeviceToDevice, st));
+  rc = dispatch_chunks(P, pl, ws, count, st, tm);
+  if (rc) return rc;
+  if (tm.collect(P->stage_ms, P->stage_launches))
+    return fail(PDAMD_EHIP, "stream failed while collecting stage timings");
+  return PDAMD_OK;
+}
+
+// pdamd_denoise in a workspace of exactly `cap` bytes at `ws`: the largest
+// chunk of patches whose layout fits.  Fails when even one patch per chunk
+// (plus the per-volume arrays) exceeds the cap.
+int denoise_capped(pdamd_problem* P, void* ws, size_t cap, void* stream) {
+  Plan pl;
+  int rc = make_plan(P, &pl);
+  if (rc) return rc;
+  if (pl.total > cap) {
+    int64_t lo = 0, hi = pl.chunk;  // lo fits (0 = nothing yet), hi does not
+    while (hi - lo > 1) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      Plan t;
+      rc = make_plan(P, &t, false, mid);
+      if (rc) return rc;
+      if (t.total <= cap) lo = mid;
+      else hi = mid;
+    }
+    if (lo == 0)
+      return fail(PDAMD_EUNSUPPORTED,
+                  "problem needs more than the %zu-byte no-allocation scratch even one patch "
+                  "at a time (volume-sized arrays are part of it)", cap);
+    rc = make_plan(P, &pl, false, lo);
+    if (rc) return rc;
+  }
+  return denoise_planned(P, &pl, ws, cap, stream);
+}
 }  // namespace pdamd_detail
 
 extern "C" {
@@ -3482,40 +3551,7 @@ int pdamd_denoise(pdamd_problem* P, void* workspace_dev, size_t workspace_bytes,
   Plan pl;
   int rc = make_plan(P, &pl);
   if (rc) return rc;
-  if (workspace_bytes < pl.total)
-    return fail(PDAMD_EWORKSPACE, "workspace %zu < %zu bytes", workspace_bytes, pl.total);
-  if (!P->input_dev || !P->output_dev || !P->weights_dev)
-    return fail(PDAMD_EINVAL, "null data pointer");
-  if ((P->method == PDAMD_HYBRID || P->method == PDAMD_ADAPTIVE) && !P->var_map_dev)
-    return fail(PDAMD_EINVAL, "hybrid / adaptive rules need a variance map");
-  if (P->method == PDAMD_ADAPTIVE && P->adaptive_method == PDAMD_ADAPT_QUT && !P->qut_tau_dev)
-    return fail(PDAMD_EINVAL, "QUT needs the per-patch threshold array");
-  if (P->lam_dev && P->method != PDAMD_OPTIMAL)
-    return fail(PDAMD_EINVAL, "a threshold scale map is only used by the optimal rule");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  char* ws = reinterpret_cast<char*>(workspace_dev);
-  const size_t esz = (P->dtype == PDAMD_F32) ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
-  HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
-  if (P->recombination == PDAMD_CENTER)  // otherwise the normalisation writes every weight
-    HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
-  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0,
-                         (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st));
-  int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
-  int64_t count = 0;
-  StageTimer tm;
-  tm.on = P->stage_ms != nullptr || P->stage_launches != nullptr;
-  tm.st = st;
-  rc = run_select(P, pl, ws, idx, &count, st, &tm);
-  if (rc) return rc;
-  P->selected = count;
-  if (P->patch_index_dev && count > 0)
-    HIP_TRY(hipMemcpyAsync(P->patch_index_dev, idx, count * sizeof(int32_t),
-                           hipMemcpyDeviceToDevice, st));
-  rc = dispatch_chunks(P, pl, ws, count, st, tm);
-  if (rc) return rc;
-  if (tm.collect(P->stage_ms, P->stage_launches))
-    return fail(PDAMD_EHIP, "stream failed while collecting stage timings");
-  return PDAMD_OK;
+  return pdamd_detail::denoise_planned(P, &pl, workspace_dev, workspace_bytes, stream);
 }
 
 int pdamd_batch_workspace_size(const pdamd_problem* probs, int count, size_t* bytes) {

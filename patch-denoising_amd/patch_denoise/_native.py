@@ -86,6 +86,8 @@ EXPORTS = (
     "pdamd_optimal_backward",
     "pdamd_noise_workspace_size",
     "pdamd_estimate_noise",
+    "pdamd_noalloc_scratch_bytes",
+    "pdamd_denoise_noalloc",
     "pdamd_last_error",
     "pdamd_version",
 )
@@ -139,6 +141,10 @@ def load_library():
                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_size_t, ctypes.c_void_p]
     lib.pdamd_estimate_noise.restype = ctypes.c_int
+    lib.pdamd_noalloc_scratch_bytes.argtypes = [ctypes.POINTER(ctypes.c_size_t)]
+    lib.pdamd_noalloc_scratch_bytes.restype = ctypes.c_int
+    lib.pdamd_denoise_noalloc.argtypes = [P, ctypes.c_void_p]
+    lib.pdamd_denoise_noalloc.restype = ctypes.c_int
     lib.pdamd_last_error.restype = ctypes.c_char_p
     lib.pdamd_version.restype = ctypes.c_char_p
     _lib = lib
