@@ -32,6 +32,7 @@
 
 #include "patch_denoise_amd.h"
 #include "pd_common.h"
+#include "pd_tridiag.h"
 
 using namespace pd;
 
@@ -3117,6 +3118,22 @@ int launch_gram_tile(const Plan& pl, const void* vol, GramArgs ga, hipStream_t s
 
 template <class E>
 int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
+  // real Grams that fit one wavefront's registers: the one-wave kernel
+  constexpr bool F64 = std::is_same<E, double>::value;
+  if constexpr (std::is_same<E, float>::value || F64) {
+    if (pl.tile && tridiag_wave_nb(pl.n, F64)) {
+      TridiagWaveArgs ta;
+      ta.H = ga.H;
+      ta.DE = ga.DE;
+      ta.TAU = ga.TAU;
+      ta.np = ga.np;
+      ta.n = pl.n;
+      if (launch_tridiag_wave(ta, F64, st))
+        return fail(PDAMD_EHIP, "tridiagonalisation launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+      return PDAMD_OK;
+    }
+  }
   switch (pl.tile) {
     case 0: return launch_tridiag_big<E>(pl, ga, st);
     case 2: return launch_tridiag<E, 8>(pl, ga, st);

@@ -1,0 +1,326 @@
+// One-wavefront Householder tridiagonalisation of real symmetric Grams
+// (n <= 8 NB), the reduction of LAPACK ?sytd2 (lower) that ssyevx / dsyevx
+// apply before their eigen stage (reference: space_time/utils.py:131-135).
+//
+// Layout: the 64 lanes form an 8 x 8 grid, lane = 8 a + b.  The matrix is cut
+// into 8 x 8 blocks; block (s, t) holds A[8s + a][8t + b] in lane (a, b).
+// Only blocks with t <= s are kept, in registers (NB (NB + 1) / 2 per lane:
+// 136 at n <= 128); diagonal blocks are kept whole (both triangles).
+//
+// Step k (reflector from column k, trailing update of A[k+1:, k+1:]):
+//   * column k is block column 0 after the shifts below, coordinate b = k % 8:
+//     its 8 owning lanes reduce |x|^2 over a (3 butterflies) and every lane
+//     forms beta, tau from the same broadcast values (identical bits);
+//   * v goes through LDS as Vt[a][s] (v at row 8s + a), so each lane reads its
+//     row-side and column-side copies as contiguous 16-value runs;
+//   * p = A v from the stored blocks: off-diagonal blocks add to the row side
+//     (p_i += A_ij v_j) and the column side (p_j += A_ij v_i), diagonal blocks
+//     to the row side only; row sides are summed over b (DPP within 8 lanes),
+//     column sides over a (butterflies across rows), and both are brought to
+//     row and column index space through LDS;
+//   * w = tau p - (tau^2 / 2) (p . v) v, zero at eliminated indices;
+//   * A -= v w^T + w v^T on the kept blocks (two FMAs per element);
+//   * after 8 steps the leading block row and column are done and every kept
+//     block moves one block up-left (register moves), so the active column
+//     block is always block 0 and all register indices are compile-time.
+// Outputs as the 4-wave kernel: v_k in packed row k of the patch's H block
+// (columns k+1 .. n-1, v_k[k+1] = 1; zeros when H_k = I), d | e | e^2, tau.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pd_common.h"
+#include "pd_tridiag.h"
+
+using namespace pd;
+
+namespace {
+
+__host__ __device__ constexpr int tri_ix(int s, int t) { return s * (s + 1) / 2 + t; }
+__host__ __device__ inline int hh_row_w(int k, int n) { return k * (n - 1) - (k * (k - 1)) / 2; }
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float x) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppf(double x) {
+  const long long v = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 8 lanes 8a .. 8a+7 (xor 1, xor 2, then the mirrored half of
+// the 8-lane group); every lane of the group gets the same bits.
+template <class R>
+__device__ __forceinline__ R sum8_b(R v) {
+  v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dppf<0x141>(v);  // row_half_mirror
+  return v;
+}
+// Sum over the 8 lanes b, b+8, ..., b+56 (xor 8 inside each 16-lane row, then
+// xor 16 and xor 32 across rows); every lane of the set gets the same bits.
+template <class R>
+__device__ __forceinline__ R sum8_a(R v) {
+  v += dppf<0x128>(v);  // row_ror:8 == xor 8 inside a 16-lane row
+  v += __shfl_xor(v, 16, kWave);
+  v += __shfl_xor(v, 32, kWave);
+  return v;
+}
+
+template <class R>
+__device__ __forceinline__ R readlane_r(R x, int l) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+  } else {
+    const long long v = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __builtin_bit_cast(R, ((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+// Wave sum, identical bits in every lane.
+template <class R>
+__device__ __forceinline__ R wave_total(R v) {
+  v = sum8_b(v);
+  v = sum8_a(v);
+  return readlane_r(v, 0);
+}
+
+// Steps [k, kend) of the reduction with NBP kept block rows / columns (a
+// compile-time count: no per-block guards).  Entered with k a multiple of 8
+// and block column 0 holding column k; shifts the kept blocks up-left after
+// every 8 steps.  Blocks past the live part (at most the last three rows of
+// the phase, left over by the shifts, and zero padding past n) only ever meet
+// v = w = 0 there, so their FMAs change nothing that is read.
+template <class R, int NB, int NBP>
+__device__ __forceinline__ void tri_phase(R (&L)[NB * (NB + 1) / 2], int& k, int kend, int n,
+                                          int& base, R* Vt, R* Pa, R* Pb, R* dd, R* ee, R* tt,
+                                          R* Hp) {
+  static_assert(NBP <= NB, "phase wider than the register block");
+  const int lane = threadIdx.x;
+  const int ra = lane >> 3, cb = lane & 7;
+  int live = NBP;  // block rows of the phase not yet shifted out
+  for (; k < kend; ++k) {
+    const int u = k & 7;  // column k = coordinate u of block column 0
+    // rows of block row s are live when s < live; in block row 0 only rows past
+    // u + over (over = 1 also drops row k + 1, which is block row 1's first
+    // row when u == 7)
+    auto row_live = [&](int s, int over) {  // over: 0 -> il > u, 1 -> il > u + 1
+      if (s == 0) return ra > u + over;
+      const bool in_phase = s + 3 < NBP ? true : s < live;
+      if (s == 1 && over == 1) return in_phase && (u < 7 || ra > 0);
+      return in_phase;
+    };
+    const R dk = readlane_r(L[tri_ix(0, 0)], 9 * u);
+    // ---- reflector from x = column k below the diagonal (lanes cb == u)
+    R sq = R(0);
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      const R x = L[tri_ix(s, 0)];
+      sq = row_live(s, 1) ? fma(x, x, sq) : sq;
+    }
+    sq = sum8_a(sq);
+    const R xn2 = readlane_r(sq, u);
+    const R alpha = (u < 7) ? readlane_r(L[tri_ix(0, 0)], 8 * (u + 1) + u)
+                            : readlane_r(L[tri_ix(1, 0)], u);
+    R beta = alpha, tk = R(0), scal = R(0);
+    if (xn2 > R(0)) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tk = (beta - alpha) / beta;
+      scal = R(1) / (alpha - beta);
+    }
+    const R one = xn2 > R(0) ? R(1) : R(0);
+    if (cb == u) {
+#pragma unroll
+      for (int s = 0; s < NBP; ++s) {
+        R v = row_live(s, 1) ? L[tri_ix(s, 0)] * scal : R(0);
+        if (s == 0) v = ra == u + 1 ? one : v;
+        if (s == 1) v = (u == 7 && ra == 0) ? one : v;
+        Vt[ra * NB + s] = v;
+      }
+    }
+    if (lane == 0) {
+      dd[k] = dk;
+      ee[k] = beta;
+      tt[k] = tk;
+    }
+    wave_sync();
+    R vr[NBP], vc[NBP];
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      vr[s] = Vt[ra * NB + s];
+      vc[s] = Vt[cb * NB + s];
+    }
+    // ---- p = A v: row sides p_i += A_ij v_j, column sides p_j += A_ij v_i
+    R pr[NBP], pc[NBP];
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      pr[s] = L[tri_ix(s, s)] * vc[s];
+#pragma unroll
+      for (int t = 0; t < s; ++t) {
+        pr[s] = fma(L[tri_ix(s, t)], vc[t], pr[s]);
+        pc[t] = (t + 1 == s) ? L[tri_ix(s, t)] * vr[s] : fma(L[tri_ix(s, t)], vr[s], pc[t]);
+      }
+    }
+    pc[NBP - 1] = R(0);
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      pr[s] = sum8_b(pr[s]);
+      pc[s] = sum8_a(pc[s]);
+    }
+    // column sides to row index space (lanes ra == 0 hold them for every cb)
+    if (ra == 0) {
+#pragma unroll
+      for (int s = 0; s < NBP; ++s) Pb[cb * NB + s] = pc[s];
+    }
+    wave_sync();
+    R q_r[NBP];
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      const R pt = pr[s] + Pb[ra * NB + s];
+      q_r[s] = row_live(s, 0) ? tk * pt : R(0);
+    }
+    // row-index values to column index space (lanes cb == 0 hold them)
+    if (cb == 0) {
+#pragma unroll
+      for (int s = 0; s < NBP; ++s) Pa[ra * NB + s] = q_r[s];
+    }
+    // K = -tau/2 (q . v) over the live indices (each index once: cb == 0)
+    R dot = R(0);
+    if (cb == 0) {
+#pragma unroll
+      for (int s = 0; s < NBP; ++s) dot = fma(q_r[s], vr[s], dot);
+    }
+    const R kc = R(-0.5) * tk * wave_total(dot);
+    wave_sync();
+    R wr[NBP], wc[NBP];
+#pragma unroll
+    for (int s = 0; s < NBP; ++s) {
+      wr[s] = fma(kc, vr[s], q_r[s]);
+      wc[s] = fma(kc, vc[s], Pa[cb * NB + s]);
+    }
+    // ---- A -= v w^T + w v^T on the kept blocks
+#pragma unroll
+    for (int s = 0; s < NBP; ++s)
+#pragma unroll
+      for (int t = 0; t <= s; ++t) {
+        R x = L[tri_ix(s, t)];
+        x = fma(-vr[s], wc[t], x);
+        x = fma(-wr[s], vc[t], x);
+        L[tri_ix(s, t)] = x;
+      }
+    // ---- v_k -> packed row k (columns k+1 .. n-1); every Gram load has returned
+    {
+      R* row = Hp + hh_row_w(k, n) - (k + 1);
+      for (int i = k + 1 + lane; i < n; i += kWave) {
+        const int il = i - base;
+        row[i] = Vt[(il & 7) * NB + (il >> 3)];
+      }
+    }
+    wave_sync();
+    if (u == 7) {  // block row / column done: shift the kept blocks up-left
+#pragma unroll
+      for (int s = 0; s + 1 < NBP; ++s)
+#pragma unroll
+        for (int t = 0; t <= s; ++t) L[tri_ix(s, t)] = L[tri_ix(s + 1, t + 1)];
+      base += 8;
+      --live;
+    }
+  }
+}
+
+// WPE: waves per SIMD the register allocation must allow (2 for float at
+// NB = 16: 256 VGPRs with a few spilled, instead of 264 and one wave).
+template <class R, int NB, int WPE>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) void tridiag_wave_kernel(TridiagWaveArgs args) {
+  constexpr int NT = NB * (NB + 1) / 2;
+  const int n = args.n;
+  const int lane = threadIdx.x;
+  const int ra = lane >> 3, cb = lane & 7;
+  extern __shared__ __align__(16) char smem[];
+  R* Vt = reinterpret_cast<R*>(smem);  // [8][NB]: v at local row 8s + a -> Vt[a * NB + s]
+  R* Pa = Vt + 8 * NB;                 // [8][NB]: q (row side), same indexing
+  R* Pb = Pa + 8 * NB;                 // [8][NB]: column-side sums, same indexing
+  R* dd = Pb + 8 * NB;                 // [8 NB]: d_k
+  R* ee = dd + 8 * NB;                 // [8 NB]: e_k (beta)
+  R* tt = ee + 8 * NB;                 // [8 NB]: tau_k
+
+  // one patch per workgroup: no patch loop, so no per-lane load offsets are
+  // hoisted and kept live across patches
+  const int p = blockIdx.x;
+  R* Hp = reinterpret_cast<R*>(args.H) + (size_t)p * n * n;
+  R L[NT];
+#pragma unroll
+  for (int s = 0; s < NB; ++s)
+#pragma unroll
+    for (int t = 0; t <= s; ++t) {
+      const int i = 8 * s + ra, j = 8 * t + cb;
+      L[tri_ix(s, t)] = (i < n && j < n) ? Hp[i * n + j] : R(0);
+    }
+  // every load of the Gram has returned before reflector rows overwrite it
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+
+  // phases of 32 steps with 16, 12, 8 and then 4 kept block rows / columns
+  int k = 0, base = 0;
+  const int last = n - 1;
+  if constexpr (NB >= 16) tri_phase<R, NB, 16>(L, k, min(32, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 12) tri_phase<R, NB, 12>(L, k, min(NB * 8 - 64, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 8) tri_phase<R, NB, 8>(L, k, min(NB * 8 - 32, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  tri_phase<R, NB, 4>(L, k, last, n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  {
+    const int il = (n - 1) - base;  // 0 .. 7
+    const R dl = readlane_r(L[tri_ix(0, 0)], 9 * il);
+    if (lane == 0) dd[n - 1] = dl;
+  }
+  wave_sync();
+  R* DEp = reinterpret_cast<R*>(args.DE) + (size_t)p * 3 * n;
+  R* TAUp = reinterpret_cast<R*>(args.TAU) + (size_t)p * n;
+  for (int q = lane; q < n; q += kWave) {
+    const R ev = q < n - 1 ? ee[q] : R(0);
+    DEp[q] = dd[q];
+    DEp[n + q] = ev;
+    DEp[2 * n + q] = ev * ev;
+    TAUp[q] = q < n - 1 ? tt[q] : R(0);
+  }
+}
+
+template <class R, int NB>
+int launch_nb(const TridiagWaveArgs& a, hipStream_t st) {
+  const size_t lds = 6 * 8 * NB * sizeof(R);
+  auto kern = tridiag_wave_kernel<R, NB, (sizeof(R) == 4 && NB == 16) ? 2 : 1>;
+  if (a.np < 1) return 0;
+  kern<<<a.np, kWave, lds, st>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+namespace pd {
+
+int tridiag_wave_nb(int n, bool f64) {
+  return n <= 32 ? 4 : (n <= 64 ? 8 : (f64 ? 0 : (n <= 96 ? 12 : (n <= 128 ? 16 : 0))));
+}
+
+int launch_tridiag_wave(const TridiagWaveArgs& a, bool f64, hipStream_t st) {
+  const int nb = tridiag_wave_nb(a.n, f64);
+  if (a.n < 2 || nb == 0) return -2;
+  if (f64) {
+    switch (nb) {
+      case 4: return launch_nb<double, 4>(a, st);
+      case 8: return launch_nb<double, 8>(a, st);
+      default: return -2;  // 136 doubles per lane do not fit: the 4-wave kernel
+    }
+  }
+  switch (nb) {
+    case 4: return launch_nb<float, 4>(a, st);
+    case 8: return launch_nb<float, 8>(a, st);
+    case 12: return launch_nb<float, 12>(a, st);
+    default: return launch_nb<float, 16>(a, st);
+  }
+}
+
+}  // namespace pd
