@@ -154,15 +154,33 @@ typedef struct pdamd_problem {
      the order of the output's float atomic additions.  No counterpart in
      the reference.                                                          */
   int launch_per_cu[4];
+
+  /* recombination strategy (results agree to float32 rounding; no
+     counterpart in the reference):
+     PDAMD_RECOMBINE_AUTO    gather when eligible, else the atomic scatter;
+     PDAMD_RECOMBINE_SCATTER every patch adds into the output with float
+                             atomics (order-dependent rounding);
+     PDAMD_RECOMBINE_GATHER  every output element sums its covering patches
+                             in increasing patch index, no atomics, results
+                             repeat bit for bit.  Eligible: float32 data, one
+                             temporal patch, more frames than patch voxels,
+                             weighted / average recombination, MP-PCA or
+                             Hybrid; otherwise PDAMD_EUNSUPPORTED.          */
+  int recombine_mode;
 } pdamd_problem;
+
+enum { PDAMD_RECOMBINE_AUTO = 0, PDAMD_RECOMBINE_SCATTER = 1, PDAMD_RECOMBINE_GATHER = 2 };
 
 /* stages for pdamd_problem.stage_ms */
 enum {
   PDAMD_STAGE_SELECT = 0,   /* mask window + select + scan + compact          */
   PDAMD_STAGE_GRAM = 1,     /* gram_kernel: gather, centre, Gram               */
   PDAMD_STAGE_EIG = 2,      /* trieig_kernel: eigenvalues, rule, eigenvectors  */
-  PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate   */
-  PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask         */
+  PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate
+                               (gather recombination: the per-patch factors) */
+  PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask
+                               (gather recombination: patch map, weight sums,
+                               gather of every output element)             */
   PDAMD_STAGE_TRIDIAG = 5,  /* tridiag_kernel: Householder tridiagonalisation  */
   PDAMD_STAGE_ADJ_CROSS = 6,    /* backward: Xc^T Gbar per patch                */
   PDAMD_STAGE_ADJ_SPECTRAL = 7, /* backward: singular-vector adjoint            */

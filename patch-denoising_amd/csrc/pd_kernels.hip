@@ -32,6 +32,7 @@
 
 #include "patch_denoise_amd.h"
 #include "pd_common.h"
+#include "pd_gather.h"
 #include "pd_tridiag.h"
 
 using namespace pd;
@@ -365,6 +366,7 @@ struct GramArgs {
   void* DE;            // np x 3n      d | e | e^2 (real type)
   void* TAU;           // np x n
   void* MEAN;          // np x pt per-frame patch means (input type), reused by the synthesis
+  void* MEANW;         // M-side: np x pt per-frame means out (gather recombination), or null
 };
 
 // Staging geometry: M-side stages kTC frames x (all rows); T-side stages kRC
@@ -508,8 +510,11 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
         }
         lds_barrier();
         if (c0 + SC < T) load(c0 + SC);
-        if (tid < tc)  // sequential over rows, as numpy's axis-0 reduction
-          mean[tid] = div_n(seq_add(Sraw[tid * LDS_W], Sraw + tid * LDS_W + 1, M - 1, 1), M);
+        if (tid < tc) {  // sequential over rows, as numpy's axis-0 reduction
+          const IN mv = div_n(seq_add(Sraw[tid * LDS_W], Sraw + tid * LDS_W + 1, M - 1, 1), M);
+          mean[tid] = mv;
+          if (a.MEANW) reinterpret_cast<IN*>(a.MEANW)[(size_t)p * T + c0 + tid] = mv;
+        }
         lds_barrier();
         for (int q = tid; q < SC * NB; q += kNT) {
           const int t = q / NB, r = q % NB;
@@ -1443,6 +1448,7 @@ struct EigArgs {
   int32_t* RANK;     // np (user array offset to chunk start) or null
   float* NOISE;      // np or null
   float* EIG;        // np x 10 or null
+  int zero_fill;     // 1: eigenvector columns ncomp .. kmax-1 of Y are set to zero
 };
 
 template <class R> struct Lim;
@@ -2363,6 +2369,11 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           if (c < kb && row_of(sl) < n) Yp[(size_t)row_of(sl) * a.kmax + c0 + c] = y[sl][c];
       wave_sync();
     }
+    if (a.zero_fill) {  // factor layout of the gather recombination: clean unused columns
+      E* Yz = reinterpret_cast<E*>(a.Y) + (size_t)p * n * a.kmax;
+      const int nz = a.kmax - ncomp;
+      for (int q = lane; q < n * nz; q += kWave) Yz[(size_t)(q / nz) * a.kmax + ncomp + q % nz] = E(R(0));
+    }
     // ---------------- per-patch scalars
     if (lane == 0) {
       a.NCOMP[p] = ncomp;
@@ -2805,12 +2816,16 @@ struct Plan {
   int etype;  // 0 float, 1 double, 2 cpx<float>, 3 cpx<double>
   int rule_f64;
   bool bf16x3 = false;  // split-BF16 Gram (FP32 data, M-side MFMA path)
+  bool gather = false;  // gather recombination (pd_gather.h) instead of the atomic scatter
   int64_t chunk;
   size_t ebytes, rbytes;
   // workspace offsets
   size_t off_MEAN = 0;
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
       off_WC, off_NC, off_TH, off_w3, off_SCR, total;
+  // gather recombination only (per call, indexed by selected-patch slot)
+  size_t off_GMAP = 0, off_GNC = 0, off_GTH = 0, off_GMEAN = 0, off_GY = 0, off_GRF = 0,
+         off_GW3 = 0;
   // backward only
   size_t off_P1 = 0, off_P2 = 0, off_PE = 0, off_GS = 0, off_S1 = 0, off_UV = 0, off_THB = 0,
          off_GH = 0;
@@ -2905,6 +2920,25 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
     pl->kmax = n;
   }
   pl->kb = std::min(pl->tile ? kKB : kKBBig, pl->kmax);
+  // recombination by gather: M-side float32 patches, one temporal patch,
+  // weighted / average, the eigen rules (10 components), not in a workspace
+  // of capped size (the per-call factor buffers would not fit)
+  if (P->recombine_mode < 0 || P->recombine_mode > 2)
+    return fail(PDAMD_EINVAL, "bad recombine_mode %d", P->recombine_mode);
+  {
+    const bool elig = !bwd && P->dtype == PDAMD_F32 && pl->etype == 0 && !pl->tside &&
+                      g.gt == 1 &&
+                      (P->recombination == PDAMD_WEIGHTED || P->recombination == PDAMD_AVERAGE) &&
+                      (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) &&
+                      pl->kmax == kGatherKM && n <= 128 &&
+                      pl->nvox * g.T < ((int64_t)1 << 31);
+    if (P->recombine_mode == PDAMD_RECOMBINE_GATHER && (!elig || chunk_cap > 0))
+      return fail(PDAMD_EUNSUPPORTED,
+                  "gather recombination needs float32 data, one temporal patch, more frames "
+                  "than patch voxels, weighted or average recombination, the MP-PCA or Hybrid "
+                  "rule and an uncapped workspace");
+    pl->gather = elig && P->recombine_mode != PDAMD_RECOMBINE_SCATTER && chunk_cap == 0;
+  }
   // chunk: keep the per-chunk buffers under ~4 GiB (6 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
@@ -2941,6 +2975,16 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
   pl->off_NC = take(C * sizeof(int));
   pl->off_TH = take(C * sizeof(double));
   pl->off_w3 = take((size_t)pl->nvox * (g.gt == 1 ? 1 : g.T) * sizeof(double));
+  if (pl->gather) {
+    const size_t NPs = (size_t)pl->npatch;
+    pl->off_GMAP = take(NPs * sizeof(int32_t));
+    pl->off_GNC = take(NPs * sizeof(int));
+    pl->off_GTH = take(NPs * sizeof(double));
+    pl->off_GMEAN = take(NPs * pt * sizeof(float));
+    pl->off_GY = take(NPs * n * kGatherKM * sizeof(float));
+    pl->off_GRF = take(NPs * (kGatherKM + 1) * pt * sizeof(float));
+    pl->off_GW3 = take((size_t)pl->nvox * sizeof(float));
+  }
   if (bwd) {
     pl->off_GH = take((size_t)pl->nvox * g.T * pl->ebytes);
     pl->off_P1 = take(C * M * n * pl->ebytes);
@@ -3230,6 +3274,7 @@ EigArgs eig_args(const pdamd_problem* P, const Plan& pl, char* ws, const GramArg
   ea.RANK = nullptr;
   ea.NOISE = nullptr;
   ea.EIG = nullptr;
+  ea.zero_fill = 0;
   return ea;
 }
 
@@ -3244,14 +3289,24 @@ GramArgs gram_args(const pdamd_problem* P, const Plan& pl, char* ws, const int32
   ga.DE = ws + pl.off_DE;
   ga.TAU = ws + pl.off_TAU;
   ga.MEAN = ws + pl.off_MEAN;
+  ga.MEANW = nullptr;
   return ga;
 }
 
 #include "pd_adjoint.inl"
 
+// Gather recombination (float32 M-side eigen rules): per chunk the Gram,
+// Householder and eigen stages write the per-call means / eigenvectors /
+// counts, the factor kernel the per-patch rows; then one pass builds the
+// patch map, the float32 weight sums and every output element.
+int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
+                      StageTimer& tm);
+
 template <class IN, class E, class RT>
 int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
                StageTimer& tm) {
+  if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value)
+    if (pl.gather) return run_chunks_gather(P, pl, ws, count, st, tm);
   const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
   double* wacc = reinterpret_cast<double*>(ws + pl.off_w3);
   const bool w3d = pl.g.gt == 1;
@@ -3304,6 +3359,75 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
                                           P->recombination, pl.nvox, pl.g.T);
   tm.end();
   HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
+}
+
+int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
+                      StageTimer& tm) {
+  const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
+  const int T = pl.g.pt;
+  GatherArgs fa;
+  fa.g = pl.g;
+  fa.shape[0] = P->shape[0];
+  fa.shape[1] = P->shape[1];
+  fa.shape[2] = P->shape[2];
+  fa.n = pl.n;
+  fa.np = 0;
+  fa.slot0 = 0;
+  fa.recomb = P->recombination;
+  fa.theta_weak = P->method == PDAMD_HYBRID ? 1 : 0;
+  fa.idx = idx_all;
+  fa.vol = reinterpret_cast<const float*>(P->input_dev);
+  fa.MEAN = reinterpret_cast<const float*>(ws + pl.off_GMEAN);
+  fa.Y = reinterpret_cast<const float*>(ws + pl.off_GY);
+  fa.WC = nullptr;
+  fa.NCOMP = reinterpret_cast<const int*>(ws + pl.off_GNC);
+  fa.THETA = reinterpret_cast<const double*>(ws + pl.off_GTH);
+  fa.RF = reinterpret_cast<float*>(ws + pl.off_GRF);
+  fa.MAP = reinterpret_cast<int32_t*>(ws + pl.off_GMAP);
+  fa.W3 = reinterpret_cast<float*>(ws + pl.off_GW3);
+  fa.mask = P->mask_dev;
+  fa.mask_kind = P->mask_kind;
+  fa.out = reinterpret_cast<float*>(P->output_dev);
+  fa.wout = P->weights_dev;
+  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
+    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+    GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
+    ga.MEANW = ws + pl.off_GMEAN + (size_t)c0 * T * sizeof(float);
+    tm.begin(PDAMD_STAGE_GRAM);
+    int rc = launch_gram_tile<float, float, false>(pl, P->input_dev, ga, st);
+    tm.end();
+    if (rc) return rc;
+    tm.begin(PDAMD_STAGE_TRIDIAG);
+    rc = launch_tridiag_tile<float>(pl, ga, st);
+    tm.end();
+    if (rc) return rc;
+    EigArgs ea = eig_args(P, pl, ws, ga);
+    ea.Y = ws + pl.off_GY + (size_t)c0 * pl.n * kGatherKM * sizeof(float);
+    ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_GNC) + c0;
+    ea.THETA = reinterpret_cast<double*>(ws + pl.off_GTH) + c0;
+    ea.zero_fill = 1;
+    ea.RANK = P->rank_dev ? P->rank_dev + c0 : nullptr;
+    ea.NOISE = P->noise_dev ? P->noise_dev + c0 : nullptr;
+    ea.EIG = P->eig_dev ? P->eig_dev + c0 * 10 : nullptr;
+    tm.begin(PDAMD_STAGE_EIG);
+    rc = launch_eig<float, float>(pl, ea, st);
+    tm.end();
+    if (rc) return rc;
+    fa.np = np;
+    fa.slot0 = (int)c0;
+    fa.WC = ea.WC;
+    tm.begin(PDAMD_STAGE_SYNTH);
+    rc = launch_gather_factors(fa, st);
+    tm.end();
+    if (rc) return fail(PDAMD_EHIP, "factor kernel launch failed");
+  }
+  tm.begin(PDAMD_STAGE_FINALIZE);
+  int rc = launch_gather_map(fa, pl.npatch, count, st);
+  if (!rc) rc = launch_gather_weights(fa, st);
+  if (!rc) rc = launch_gather_recombine(fa, st);
+  tm.end();
+  if (rc) return fail(PDAMD_EHIP, "gather recombination launch failed");
   return PDAMD_OK;
 }
 
@@ -3489,11 +3613,13 @@ int denoise_planned(pdamd_problem* P, const void* plan, void* workspace_dev,
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   char* ws = reinterpret_cast<char*>(workspace_dev);
   const size_t esz = (P->dtype == PDAMD_F32) ? 4 : (P->dtype == PDAMD_C128 ? 16 : 8);
-  HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
-  if (P->recombination == PDAMD_CENTER)  // otherwise the normalisation writes every weight
-    HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
-  HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0,
-                         (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st));
+  if (!pl.gather) {  // the gather recombination writes every output element and weight
+    HIP_TRY(hipMemsetAsync(P->output_dev, 0, (size_t)pl.nvox * pl.g.T * esz, st));
+    if (P->recombination == PDAMD_CENTER)  // otherwise the normalisation writes every weight
+      HIP_TRY(hipMemsetAsync(P->weights_dev, 0, (size_t)pl.nvox * pl.g.T * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(ws + pl.off_w3, 0,
+                           (size_t)pl.nvox * (pl.g.gt == 1 ? 1 : pl.g.T) * sizeof(double), st));
+  }
   int32_t* idx = reinterpret_cast<int32_t*>(ws + pl.off_idx);
   int64_t count = 0;
   StageTimer tm;

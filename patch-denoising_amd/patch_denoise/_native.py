@@ -22,6 +22,7 @@ LOSS = {"fro": 0, "nuc": 1, "ope": 2}
 RECOMB = {"weighted": WEIGHTED, "average": AVERAGE, "center": CENTER}
 MASK_NONE, MASK_SPATIAL, MASK_FULL = 0, 1, 2
 PREC = {"auto": 0, "fp32": 1, "fp64": 2, "bf16x3": 3}
+RECOMBINE = {"auto": 0, "scatter": 1, "gather": 2}
 
 _DTYPES = {
     np.dtype(np.float32): F32,
@@ -69,6 +70,7 @@ class Problem(ctypes.Structure):
         ("tau0_log", ctypes.c_double),
         ("qut_tau_dev", ctypes.c_void_p),
         ("launch_per_cu", ctypes.c_int * 4),
+        ("recombine_mode", ctypes.c_int),
     ]
 
 

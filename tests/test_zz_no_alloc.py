@@ -4,10 +4,13 @@ pdamd_denoise (caller workspace), through the C ABI.
 Two calls in a row share the GPU's scratch array: the headline 80x80x48x300
 MP-PCA volume (processed in scratch-sized chunks), then a smaller Hybrid-PCA
 volume with a mask.  Each must equal the allocating entry point's result:
-identical selected patches and ranks, output rel-L2 <= 1e-6 (only the order of
-the float additions of overlapping patches differs), weights within 1e-7.  A
+identical selected patches and ranks, output and weights within rel-L2 1e-6
+(only the order / precision of the additions of overlapping patches differs).  A
 problem whose per-volume arrays alone exceed the scratch is refused before
-any GPU work.
+any GPU work.  Where the gather recombination's per-call factor buffers do
+not fit the scratch, the no-allocation path recombines with the atomic
+scatter: results then agree to float32 rounding of the weight sums (the
+gather sums them in patch order in float32, the scatter in float64).
 """
 
 import ctypes
@@ -56,7 +59,7 @@ def _compare(tag, got, ref):
     e, ew = rel_l2(out, rout), rel_l2(w, rw)
     print(f"[noalloc] {tag}: {len(info['index'])} patches, rel-L2 out {e:.2e} weights {ew:.2e}",
           flush=True)
-    assert e <= 1e-6 and ew <= 1e-7, (tag, e, ew)
+    assert e <= 1e-6 and ew <= 1e-6, (tag, e, ew)
 
 
 def test_noalloc_matches_allocating_entry_point():
