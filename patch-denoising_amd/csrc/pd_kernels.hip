@@ -1769,6 +1769,56 @@ __device__ double local_min(F f, double x0, double* fbest) {
   return bx;
 }
 
+// Eigenvector of the tridiagonal (d, e; e2 = e^2) for the eigenvalue lm by a
+// twisted factorization (the eigenvector step of the MRRR algorithm, Dhillon
+// & Parlett 2004), one lane: top-down pivots D+ of L+ D+ L+^T = T - lm I into
+// z[i * ks], bottom-up pivots of U- D- U-^T (multipliers u into um[i * ks]),
+// twist index r = argmin |gamma_r|, gamma_r = D+_r + D-_r - (d_r - lm); then
+// z_r = 1, z_i = -(e_i / D+_i) z_{i+1} (i < r; each z_i replaces D+_i once
+// read), z_i = -u_i z_{i-1} (i > r); unit 2-norm.  Pivots are kept at least
+// eps_piv in magnitude.  False when the result is not finite.
+template <class R>
+__device__ bool twisted_vec(const R* d, const R* e, const R* e2, int n, R lm, R eps_piv, R* z,
+                            R* um, int ks) {
+  auto guard = [&](R x) { return fabs(x) < eps_piv ? copysign(eps_piv, x) : x; };
+  R dv = guard(d[0] - lm);
+  z[0] = dv;
+  for (int i = 0; i + 1 < n; ++i) {
+    dv = guard((d[i + 1] - lm) - fdiv_(e2[i], dv));
+    z[(i + 1) * ks] = dv;
+  }
+  R dm = guard(d[n - 1] - lm);
+  R best = fabs(dv + dm - (d[n - 1] - lm));
+  int r = n - 1;
+  for (int i = n - 2; i >= 0; --i) {
+    const R u = fdiv_(e[i], dm);
+    um[(i + 1) * ks] = u;
+    dm = guard((d[i] - lm) - u * e[i]);
+    const R g = fabs(z[i * ks] + dm - (d[i] - lm));
+    if (g < best) {
+      best = g;
+      r = i;
+    }
+  }
+  R zi = R(1), ss = R(1);
+  for (int i = r - 1; i >= 0; --i) {
+    zi = -fdiv_(e[i], z[i * ks]) * zi;
+    z[i * ks] = zi;
+    ss += zi * zi;
+  }
+  z[r * ks] = R(1);
+  zi = R(1);
+  for (int i = r + 1; i < n; ++i) {
+    zi = -um[i * ks] * zi;
+    z[i * ks] = zi;
+    ss += zi * zi;
+  }
+  if (!(ss < R(INFINITY))) return false;
+  const R inv = R(1) / sqrt(ss);
+  for (int i = 0; i < n; ++i) z[i * ks] *= inv;
+  return true;
+}
+
 // Every eigenvector of the tridiagonal (d, e), eigenvalues lam[0..n) in
 // ascending order, into Z (row stride ks, column c = vector c).  Lane c of the
 // wavefront handles component c (64 at a time) with a twisted factorization
@@ -1905,13 +1955,26 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   double* wc = lds_carve<double>(cur, n + 16);
   double* sv = AD ? lds_carve<double>(cur, n + 16) : nullptr;  // ADAPTIVE: singular values, descending
   R* zb = lds_carve<R>(cur, (size_t)n * KB);  // z, component-major per lane: zb[i * KB + c]
+  R* umb = lds_carve<R>(cur, (size_t)n * KB);  // twisted-factorization multipliers, same layout
+  R* ld = lds_carve<R>(cur, n + 16);           // the patch's tridiagonal d | e | e^2 in LDS
+  R* le = lds_carve<R>(cur, n + 16);
+  R* le2 = lds_carve<R>(cur, n + 16);
   __shared__ int sh_i[4];
   __shared__ double sh_d[4];
 
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
-    const R* __restrict__ dg = reinterpret_cast<const R*>(a.DE) + (size_t)p * 3 * n;
-    const R* __restrict__ eg = dg + n;
-    const R* __restrict__ e2g = dg + 2 * n;
+    {  // the tridiagonal into LDS: every recurrence below reads it
+      const R* src = reinterpret_cast<const R*>(a.DE) + (size_t)p * 3 * n;
+      for (int i = lane; i < n; i += kWave) {
+        ld[i] = src[i];
+        le[i] = src[n + i];
+        le2[i] = src[2 * n + i];
+      }
+      wave_sync();
+    }
+    const R* dg = ld;
+    const R* eg = le;
+    const R* e2g = le2;
     // Gershgorin interval
     R glo = R(INFINITY), ghi = R(-INFINITY), emax2 = R(0);
     for (int i = lane; i < n; i += kWave) {
@@ -2224,14 +2287,27 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         const R inv = mx > R(0) ? R(1) / mx : R(1);
         for (int i = 0; i < n; ++i) zb[i * KB + c] *= inv;
       };
+      // eigenvectors by twisted factorization, one lane per component; a
+      // non-finite result falls back to inverse iteration from a fixed start
       if (lane < kb) {
-        uint32_t h = 0x9E3779B9u * (uint32_t)(c0 + lane + 1);
-        for (int i = 0; i < n; ++i) {
-          h ^= h << 13; h ^= h >> 17; h ^= h << 5;
-          zb[i * KB + lane] = R(0.5) + (R)(h & 0xFFFF) * R(1.0 / 65536.0);
+        if (!twisted_vec(dg, eg, e2g, n, lam[c0 + lane], eps_piv, zb + lane, umb + lane, KB)) {
+          uint32_t h = 0x9E3779B9u * (uint32_t)(c0 + lane + 1);
+          for (int i = 0; i < n; ++i) {
+            h ^= h << 13; h ^= h >> 17; h ^= h << 5;
+            zb[i * KB + lane] = R(0.5) + (R)(h & 0xFFFF) * R(1.0 / 65536.0);
+          }
+          for (int it = 0; it < n_init; ++it) solve(lane);
         }
-        for (int it = 0; it < n_init; ++it) solve(lane);
       }
+      // components of this batch inside an eigenvalue cluster (closer than
+      // clus to another kept component): they get a second orthogonalisation
+      // pass after one more inverse-iteration step
+      bool clustered = false;
+      if (lane < kb) {
+        for (int q = 0; q < ncomp; ++q)
+          if (q != c0 + lane && fabs(lam[q] - lam[c0 + lane]) <= clus) clustered = true;
+      }
+      const bool any_cluster = __any(clustered);
       wave_sync();
       // rows -> registers
       R z[NR][KBC];
@@ -2287,13 +2363,14 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
           for (int sl = 0; sl < NR; ++sl) z[sl][c] *= inv;
         }
         if (pass == 0) {
+          if (!any_cluster) break;  // isolated eigenvalues: the vectors are final
 #pragma unroll
           for (int sl = 0; sl < NR; ++sl)
 #pragma unroll
             for (int c = 0; c < KBC; ++c)
               if (c < kb && row_of(sl) < n) zb[row_of(sl) * KB + c] = z[sl][c];
           wave_sync();
-          if (lane < kb) solve(lane);
+          if (lane < kb && clustered) solve(lane);
           wave_sync();
           load_z();
         }
@@ -2394,7 +2471,8 @@ size_t trieig_lds_bytes(int n, int kb, bool ad = false) {
   s += al((n + 16) * sizeof(int)) + al((n + 16) * sizeof(R));
   s += al((n + 16) * sizeof(R)) + al((n + 16) * sizeof(int)) +
        (ad ? 2 : 1) * al((n + 16) * sizeof(double));
-  s += al((size_t)n * kb * sizeof(R));
+  s += 2 * al((size_t)n * kb * sizeof(R));       // zb, umb
+  s += 3 * al((size_t)(n + 16) * sizeof(R));      // d, e, e^2
   return s;
 }
 
