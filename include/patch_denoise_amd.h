@@ -179,13 +179,15 @@ enum {
   PDAMD_STAGE_SYNTH = 3,    /* synth_kernel: projection + scatter-accumulate
                                (gather recombination: the per-patch factors) */
   PDAMD_STAGE_FINALIZE = 4, /* finalize_kernel: normalisation and mask
-                               (gather recombination: patch map, weight sums,
-                               gather of every output element)             */
+                               (gather recombination: patch map and weight
+                               sums)                                       */
   PDAMD_STAGE_TRIDIAG = 5,  /* tridiag_kernel: Householder tridiagonalisation  */
   PDAMD_STAGE_ADJ_CROSS = 6,    /* backward: Xc^T Gbar per patch                */
   PDAMD_STAGE_ADJ_SPECTRAL = 7, /* backward: singular-vector adjoint            */
   PDAMD_STAGE_ADJ_SYNTH = 8,    /* backward: dL/dX scatter-accumulate           */
-  PDAMD_NSTAGES = 9
+  PDAMD_STAGE_RECOMBINE = 9,    /* gather recombination: every output element
+                                   summed from its covering patches' factors */
+  PDAMD_NSTAGES = 10
 };
 
 /* Bytes of device workspace pdamd_denoise needs for this problem. */

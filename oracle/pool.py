@@ -1,4 +1,8 @@
-"""Run the CPU oracle over many patches in parallel (test helper).
+"""Run the CPU oracle over many patches in parallel.
+
+TEST INFRASTRUCTURE ONLY, like the rest of ``oracle/``: used by the tests to
+check whole volumes and by the benchmark's CPU-baseline leg to time the
+oracle on one process per host core.
 
 The GPU tests denoise whole volumes; checking them against the oracle patch by
 patch takes minutes on one core, so the work is spread over a pool of worker
@@ -200,9 +204,65 @@ def _window_task(spec, ps, po, method, params, indices, lo, hi):
 # ---------------------------------------------------------------------------
 
 
-def _pool():
-    return ProcessPoolExecutor(max_workers=N_WORKERS, mp_context=mp.get_context("spawn"),
-                               initializer=_init_worker)
+def _pool(workers=None):
+    return ProcessPoolExecutor(max_workers=workers or N_WORKERS,
+                               mp_context=mp.get_context("spawn"), initializer=_init_worker)
+
+
+def _warm_task(_):
+    from oracle import cpu_ref as O
+
+    rng = np.random.default_rng(0)
+    O.mppca_patch(rng.standard_normal((27, 40)).astype(np.float32), 1.0)
+    return os.getpid()
+
+
+def _mppca_time_task(spec, ps, po, ts, indices):
+    """MP-PCA of the given patches with weighted accumulation into a
+    worker-local volume, as the reference's patch loop does
+    (space_time/base.py:96-110, lowrank.py:24-63); returns the seconds spent."""
+    import time
+
+    from oracle import cpu_ref as O
+
+    data = _open(spec)
+    src = O.StridedPatches(data, ps, po)
+    out = np.zeros(data.shape, np.float32)
+    w = np.zeros(data.shape, np.float32)
+    dst, wts = O.StridedPatches(out, ps, po), O.StridedPatches(w, ps, po)
+    M = int(np.prod(ps[:-1]))
+    t0 = time.perf_counter()
+    for L in indices:
+        X = np.array(src.get(int(L))).reshape(M, -1)
+        P, r, _ = O.mppca_patch(X, ts)
+        th = 1 / (2 + r)
+        o = dst.get(int(L))
+        o += P.reshape(ps) * th
+        ww = wts.get(int(L))
+        ww += th
+    return time.perf_counter() - t0
+
+
+def time_mppca_pool(vol, ps, po, ts, indices, workers=None):
+    """Wall time of MP-PCA over ``indices`` on a pool of ``workers`` processes
+    (one BLAS thread each), the pool started and warmed beforehand.  The
+    final sum of the workers' partial volumes is not included."""
+    import time
+
+    workers = workers or N_WORKERS
+    indices = np.asarray(indices, np.int64)
+    parts = np.array_split(indices, max(1, min(len(indices), 4 * workers)))
+    pool = _pool(workers)
+    try:
+        list(pool.map(_warm_task, range(4 * workers)))
+        t0 = time.perf_counter()
+        futs = [pool.submit(_mppca_time_task, vol.spec(), ps, po, ts, p) for p in parts
+                if len(p)]
+        busy = sum(f.result() for f in futs)
+        wall = time.perf_counter() - t0
+    finally:
+        pool.shutdown()
+    return dict(wall_s=wall, busy_s=busy, workers=workers, patches=int(len(indices)))
 
 
 def oracle_ranks(vol, ps, po, method, params, indices, pool=None):

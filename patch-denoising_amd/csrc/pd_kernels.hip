@@ -3503,8 +3503,12 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
   tm.begin(PDAMD_STAGE_FINALIZE);
   int rc = launch_gather_map(fa, pl.npatch, count, st);
   if (!rc) rc = launch_gather_weights(fa, st);
-  if (!rc) rc = launch_gather_recombine(fa, st);
   tm.end();
+  if (!rc) {
+    tm.begin(PDAMD_STAGE_RECOMBINE);
+    rc = launch_gather_recombine(fa, st);
+    tm.end();
+  }
   if (rc) return fail(PDAMD_EHIP, "gather recombination launch failed");
   return PDAMD_OK;
 }

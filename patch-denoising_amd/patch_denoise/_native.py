@@ -75,7 +75,7 @@ class Problem(ctypes.Structure):
 
 
 STAGES = ("select", "gram", "trieig", "synth", "finalize", "tridiag", "adj_cross", "adj_spectral",
-          "adj_synth")
+          "adj_synth", "recombine")
 
 
 EXPORTS = (

@@ -11,7 +11,7 @@ batched device call replayed from its hipGraph.  For every volume:
   with the GPU output (rel-L2 <= 1e-5) and weights (<= 1e-6);
 * the retained rank of a strided sample of selected patches equals the
   oracle's, except where the oracle's own decision margin shows a tie at the
-  cutoff (``tests/oracle_pool.py``).
+  cutoff (``oracle/pool.py``).
 
 References: space_time/base.py:177-270 (driver, recombination),
 lowrank.py:42-63 (MP-PCA), :104-119 (Hybrid), :170-184 (raw SVT),
@@ -21,7 +21,7 @@ lowrank.py:42-63 (MP-PCA), :104-119 (Hybrid), :170-184 (raw SVT),
 import numpy as np
 import pytest
 
-from tests import oracle_pool as OP
+from oracle import pool as OP
 
 pytestmark = pytest.mark.gpu
 

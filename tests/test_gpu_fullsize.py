@@ -5,7 +5,7 @@ small window of voxels, every selected patch covering the window and the
 reference's recombination (weighted: theta = 1/(2+r), out / w) on those
 voxels only.  The window output and weights must match, and the retained rank
 of every selected patch must equal the oracle's (computed by a CPU process
-pool, tests/oracle_pool.py) except at ties: where the oracle's own decision
+pool, oracle/pool.py) except at ties: where the oracle's own decision
 margin is within float32 rounding of the patch's largest eigenvalue.
 """
 
@@ -76,7 +76,7 @@ def test_mppca_80x80x48x300_window_and_ranks():
     # every selected patch's rank against the oracle (CPU process pool); a
     # mismatch is allowed only at a tie: the oracle's own decision margin
     # within float32 rounding of the largest eigenvalue
-    from tests import oracle_pool as OP
+    from oracle import pool as OP
 
     with OP.SharedVolume(data) as sv:
         r_or, margin = OP.oracle_ranks(sv, ps, po, "mppca", {"ts": 1.0}, sel)
@@ -113,7 +113,7 @@ def test_optimal_fro_64x64x32x120_window():
     print("[fullsize] optimal-fro 64x64x32x120 gpu run", flush=True)
     out, w, info = _run_gpu(den, data, None)
     sel = info["index"].astype(np.int64)
-    from tests import oracle_pool as OP
+    from oracle import pool as OP
 
     with OP.SharedVolume(data) as sv:
         r_or, margin = OP.oracle_ranks(sv, ps, po, "optimal",

@@ -1,9 +1,9 @@
-"""The parallel oracle helper (tests/oracle_pool.py) against the oracle's own
+"""The parallel oracle helper (oracle/pool.py) against the oracle's own
 sequential patch driver on small volumes: same ranks, same window output."""
 
 import numpy as np
 
-from tests import oracle_pool as OP
+from oracle import pool as OP
 
 
 def test_pool_matches_sequential_oracle():
