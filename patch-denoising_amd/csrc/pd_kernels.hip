@@ -741,7 +741,8 @@ size_t tridiag_lds_bytes(int n) {
 //                       in row k, columns k+1 .. n-1, a part of the matrix
 //                       that no later step reads
 // ---------------------------------------------------------------------------
-constexpr int kNBig = 512;  // largest supported Gram size
+constexpr int kNBig = 2048;  // largest supported Gram size
+constexpr int kNMid = 512;   // largest Gram size of the 8-rows-per-lane eigenvector kernel
 constexpr int kGBT = 64;    // Gram output tile
 constexpr int kGBK = 32;    // reduction indices per staged chunk
 
@@ -2390,6 +2391,28 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
       for (int sl = 0; sl < NR; ++sl)
 #pragma unroll
         for (int c = 0; c < KBC; ++c) y[sl][c] = E(z[sl][c]);
+      if constexpr (NR > kNMid / kWave) {
+        // Grams above kNMid (one component per batch): each reflector row is
+        // read once from memory into registers, no prefetch
+        static_assert(KBC == 1, "one component per batch above kNMid");
+        for (int k = n - 2; k >= 0; --k) {
+          const E tk = TAUp[k];
+          if (re_(tk) == R(0) && im_(tk) == R(0)) continue;
+          const E* row = Hp + (size_t)k * n;
+          E v[NR];
+#pragma unroll
+          for (int sl = 0; sl < NR; ++sl) {
+            const int i = row_of(sl);
+            v[sl] = (i > k + 1 && i < n) ? row[i] : E(R(i == k + 1 ? 1 : 0));
+          }
+          E t = conj_(v[0]) * y[0][0];
+#pragma unroll
+          for (int sl = 1; sl < NR; ++sl) t += conj_(v[sl]) * y[sl][0];
+          const E sdot = tk * dpp_sum(t);
+#pragma unroll
+          for (int sl = 0; sl < NR; ++sl) y[sl][0] -= v[sl] * sdot;
+        }
+      } else {
       constexpr int PF = NR > 2 ? 2 : 8;
       E vr[NR][PF], tv[PF];
       auto fetch = [&](int kbase) {  // reflectors kbase, kbase-1, ..., kbase-PF+1
@@ -2438,6 +2461,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
             for (int sl = 0; sl < NR; ++sl) y[sl][c] -= cv[sl][u] * sdot;
           }
         }
+      }
       }
 #pragma unroll
       for (int sl = 0; sl < NR; ++sl)
@@ -2997,7 +3021,7 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
       return fail(PDAMD_EUNSUPPORTED, "the backward pass supports min(patch voxels, frames) <= 128");
     pl->kmax = n;
   }
-  pl->kb = std::min(pl->tile ? kKB : kKBBig, pl->kmax);
+  pl->kb = std::min(pl->tile ? kKB : (n > kNMid ? 1 : kKBBig), pl->kmax);
   // recombination by gather: M-side float32 patches, one temporal patch,
   // weighted / average, the eigen rules (10 components), not in a workspace
   // of capped size (the per-call factor buffers would not fit)
@@ -3272,8 +3296,11 @@ int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
   auto kern = pl.tile ? (ad ? trieig_kernel<E, RT, 2, kKB, true> : trieig_kernel<E, RT>)
               : pl.n <= 4 * kWave
                   ? (ad ? trieig_kernel<E, RT, 4, kKBBig, true> : trieig_kernel<E, RT, 4, kKBBig>)
-                  : (ad ? trieig_kernel<E, RT, kNBig / kWave, kKBBig, true>
-                        : trieig_kernel<E, RT, kNBig / kWave, kKBBig>);
+              : pl.n <= kNMid
+                  ? (ad ? trieig_kernel<E, RT, kNMid / kWave, kKBBig, true>
+                        : trieig_kernel<E, RT, kNMid / kWave, kKBBig>)
+                  : (ad ? trieig_kernel<E, RT, kNBig / kWave, 1, true>
+                        : trieig_kernel<E, RT, kNBig / kWave, 1>);
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = pick_per_cu(pl.per_cu[2], 16, lds);
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
@@ -3291,8 +3318,9 @@ int launch_eig_all(const Plan& pl, EigArgs ea, hipStream_t st) {
 }
 
 constexpr int kSynSCBig = 8, kSynKGBig = 4;  // synth chunking for Grams above 128
-static_assert(kSynSC * 128 <= kSynNI * kNT && kSynSCBig * kNBig <= kSynNI * kNT &&
-                  2 * kSynSCBig * 256 <= kSynNI * kNT,
+constexpr int kSynSCHuge = 2;                // ... and for Grams above kNMid
+static_assert(kSynSC * 128 <= kSynNI * kNT && kSynSCBig * kNMid <= kSynNI * kNT &&
+                  2 * kSynSCBig * 256 <= kSynNI * kNT && kSynSCHuge * kNBig <= kSynNI * kNT,
               "one staged synthesis chunk must fit kSynNI items per thread");
 
 template <class IN, class E, bool TSIDE>
@@ -3303,12 +3331,23 @@ int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) 
   const size_t lds_mid = synth_lds_bytes<IN, E, TSIDE, 2 * kSynSCBig, 2 * kSynKGBig>(
       pl.n, pl.kmax, pl.g.M, pl.g.pt);
   const bool mid = big && pl.n <= 256 && lds_mid <= 160 * 1024;
+  // Grams above kNMid: 2-row / -frame chunks, groups of 4 components (2 when
+  // 4 overflow the LDS, as for complex float64 near the maximum size)
+  const bool huge = big && pl.n > kNMid;
+  const size_t lds_h4 = synth_lds_bytes<IN, E, TSIDE, kSynSCHuge, kSynKGBig>(
+      pl.n, pl.kmax, pl.g.M, pl.g.pt);
+  const bool h2 = huge && lds_h4 > 160 * 1024;
   const size_t lds = mid ? lds_mid
+                     : h2 ? synth_lds_bytes<IN, E, TSIDE, kSynSCHuge, kSynKGBig / 2>(
+                                pl.n, pl.kmax, pl.g.M, pl.g.pt)
+                     : huge ? lds_h4
                      : big ? synth_lds_bytes<IN, E, TSIDE, kSynSCBig, kSynKGBig>(pl.n, pl.kmax, pl.g.M, pl.g.pt)
                            : synth_lds_bytes<IN, E, TSIDE>(pl.n, pl.kmax, pl.g.M, pl.g.pt);
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "synth LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = mid ? synth_kernel<IN, E, TSIDE, 2 * kSynSCBig, 2 * kSynKGBig>
-                  : big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
+              : h2 ? synth_kernel<IN, E, TSIDE, kSynSCHuge, kSynKGBig / 2>
+              : huge ? synth_kernel<IN, E, TSIDE, kSynSCHuge, kSynKGBig>
+              : big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int per_cu = pick_per_cu(pl.per_cu[3], 8, lds);
   kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);

@@ -246,3 +246,37 @@ def golden_volumes():
         sigma_map=3.0 * (1 + 0.1 * rng.random(shape)), raw_tau=3.0 * (np.sqrt(27) + np.sqrt(22)),
     )
     return vols
+
+
+def golden_volumes_small_m():
+    """Tiny seeded volumes whose patches hold fewer than 10 voxels (M < 10):
+    the eigenvalue rules then see fewer than 10 nonzero Gram eigenvalues.
+
+    * ``2d_p3``: 2-D 14x12 x 30 frames, patch 3, overlap 2 (M = 9);
+    * ``3d_p2``: 3-D 8x8x6 x 24 frames, patch 2, overlap 1 (M = 8).
+    Rank-2 signal plus Gaussian noise; optimal shrinkage is excluded (the
+    reference raises when the patch has fewer voxels than frames)."""
+    rng = np.random.default_rng(12)
+
+    def lowrank(shape, T, rank, amp, base, sigma):
+        maps = rng.standard_normal((rank,) + shape)
+        tcs = rng.standard_normal((rank, T))
+        clean = base + amp * np.tensordot(maps, tcs, axes=([0], [0]))
+        return (clean + sigma * rng.standard_normal(shape + (T,))).astype(np.float32)
+
+    vols = {}
+    shape = (14, 12)
+    vols["2d_p3"] = dict(
+        data=lowrank(shape, 30, 2, 5.0, 40.0, 2.0), mask=None,
+        patch_shape=3, patch_overlap=2, sigma=2.0,
+        sigma_map=2.0 * (1 + 0.1 * rng.random(shape)), raw_tau=2.0 * (3.0 + np.sqrt(30)),
+        optimal_ok=False,
+    )
+    shape = (8, 8, 6)
+    vols["3d_p2"] = dict(
+        data=lowrank(shape, 24, 2, 5.0, 40.0, 2.0), mask=None,
+        patch_shape=2, patch_overlap=1, sigma=2.0,
+        sigma_map=2.0 * (1 + 0.1 * rng.random(shape)), raw_tau=2.0 * (np.sqrt(8) + np.sqrt(24)),
+        optimal_ok=False,
+    )
+    return vols

@@ -14,6 +14,8 @@ copied: the fixtures hold inputs and the reference's outputs only.
 Fixtures written (all small, ``np.savez_compressed``):
 
 * ``e2e_<case>.npz``   end-to-end ``denoise()`` outputs per method/recombination
+                       (``e2e_small_m``: the volumes with fewer than 10 voxels
+                       per patch, ``workloads.golden_volumes_small_m``)
 * ``patches_cfg1.npz`` per-patch eigen / SVD analysis + rank decisions on
                        Casorati matrices of the 32x32x16x50 workload
 * ``mp_median.npz``    Marchenko-Pastur median on a beta x eps grid
@@ -73,11 +75,11 @@ def _run(fn, data, **kw):
     return out[0], out[1]
 
 
-def e2e_cases():
+def e2e_cases(vols=None):
     """Small volumes x every hot-path method x recombination."""
     from cases import run_table
 
-    vols = workloads.golden_volumes()
+    vols = vols or workloads.golden_volumes()
     for name, spec in vols.items():
         data = spec["data"]
         mask = spec.get("mask")
@@ -286,7 +288,8 @@ if __name__ == "__main__":
     import warnings
 
     warnings.simplefilter("ignore")
-    which = sys.argv[1:] or ["e2e", "patches", "mpm", "tau", "cfg1", "noise", "adaptive"]
+    which = sys.argv[1:] or ["e2e", "e2e_small_m", "patches", "mpm", "tau", "cfg1", "noise",
+                             "adaptive"]
     if "mpm" in which:
         mp_median_grid()
     if "tau" in which:
@@ -295,6 +298,8 @@ if __name__ == "__main__":
         per_patch_cfg1()
     if "e2e" in which:
         e2e_cases()
+    if "e2e_small_m" in which:
+        e2e_cases(workloads.golden_volumes_small_m())
     if "cfg1" in which:
         cfg1_full()
     if "noise" in which:

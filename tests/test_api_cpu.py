@@ -45,8 +45,11 @@ def test_workspace_size_and_validation_without_gpu():
     large = N.make_problem((20, 20, 20, 300), (6, 6, 6, 300), (1, 1, 1, 300), np.float32, N.MPPCA)
     assert N.workspace_bytes(large) > 0  # n = min(216, 300): large-Gram kernels
     big = N.make_problem((20, 20, 20, 600), (9, 9, 9, 600), (1, 1, 1, 600), np.float32, N.MPPCA)
-    with pytest.raises(N.NativeError):
-        N.workspace_bytes(big)  # n = min(729, 600) > 512
+    assert N.workspace_bytes(big) > 0  # n = min(729, 600) = 600: above 512
+    huge = N.make_problem((20, 20, 20, 2500), (13, 13, 13, 2500), (1, 1, 1, 2500), np.float32,
+                          N.MPPCA)
+    with pytest.raises(N.NativeError, match="supported maximum"):
+        N.workspace_bytes(huge)  # n = min(2197, 2500) > 2048
     short = N.make_problem((8, 8, 8, 8), (3, 3, 3, 8), (1, 1, 1, 8), np.float32, N.MPPCA)
     with pytest.raises(ValueError):
         N.workspace_bytes(short)  # eigen rules need >= 10 frames

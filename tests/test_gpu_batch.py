@@ -88,6 +88,7 @@ def test_batch_reports_unsupported_volume():
 
     rng = np.random.default_rng(0)
     ok = rng.standard_normal((8, 8, 8, 30)).astype(np.float32) + 10
-    big = rng.standard_normal((10, 10, 10, 600)).astype(np.float32) + 10  # n = min(729, 600)
+    # n = min(13^3, 2200) = 2197: above the largest supported Gram (2048)
+    big = rng.standard_normal((13, 13, 13, 2200)).astype(np.float32) + 10
     with pytest.raises(N.NativeError, match="volume 1"):
-        denoise_batch([("mp_pca", (ok, 3, 2), {}), ("mp_pca", (big, 9, 8), {})])
+        denoise_batch([("mp_pca", (ok, 3, 2), {}), ("mp_pca", (big, 13, 12), {})])

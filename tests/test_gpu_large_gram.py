@@ -2,7 +2,8 @@
 
 Patch shapes of the full-size workloads whose Gram exceeds 128 -- 11^3 x 200
 complex NORDIC patches, 7^3-voxel patches of 200 to 400 frames, 6^3 x 300,
-15^2 x 300 in 2-D -- on volumes just a few patches wide, so that the oracle
+15^2 x 300 in 2-D, and Grams above 512 up to 11^3 = 1331 voxels over 1400
+frames -- on volumes just a few patches wide, so that the oracle
 (a NumPy restatement of the reference, SVD / eigh per patch) finishes in
 seconds.  Both Gram sides are covered: T-side (frames < voxels, G = Xc^H Xc)
 and M-side (voxels <= frames, G = Xc Xc^H).
@@ -71,6 +72,23 @@ def _cases():
     d = workloads.config3(shape=(12, 12, 11), T=200)
     c.append(("nordic_c64_p11_T200", "nordic", d, (11, 10),
               {"noise_std": 25.0, "n_iter_threshold": 1}))
+    # Grams above 512 (one eigenvector component per batch, the Householder
+    # reduction mostly on the matrix in global memory):
+    # M-side n = 648 (9x9x8 voxels < 700 frames), MP-PCA float32
+    c.append(("mppca_f32_p9x9x8_T700", "mp_pca", _lowrank((10, 10, 9), 700, 5, 7),
+              ((9, 9, 8), (8, 8, 7)), {}))
+    # T-side n = 520 (11^3 voxels > 520 frames), complex64 raw SVT, FP64 eigen
+    # stage (NORDIC's threshold draw alone, a 692,120 x 520 SVD, would take
+    # the oracle a minute here)
+    d = workloads.config3(shape=(12, 11, 11), T=520)
+    c.append(("raw_c64_p11_T520", "raw_svt", d, (11, 10),
+              {"threshold": 25.0 * np.sqrt(2) * (np.sqrt(1331) + np.sqrt(520))}))
+    # M-side n = 1331 (11^3 voxels < 1400 frames), MP-PCA float32: the
+    # reference command line's default patch with a long series
+    c.append(("mppca_f32_p11_T1400", "mp_pca", _lowrank((12, 11, 11), 1400, 5, 8), (11, 10), {}))
+    # T-side n = 700, optimal shrinkage (Frobenius), float32
+    c.append(("optimal_fro_f32_p11_T700", "optimal_thresholding",
+              _lowrank((12, 11, 11), 700, 6, 9), (11, 10), {"loss": "fro"}))
     return c
 
 
@@ -83,7 +101,8 @@ def test_large_gram_matches_oracle(case):
     from oracle import cpu_ref as O
 
     name, fn, data, (ps, po), kw = case
-    n = min(ps ** (data.ndim - 1), data.shape[-1])
+    vox = int(np.prod(ps)) if isinstance(ps, tuple) else ps ** (data.ndim - 1)
+    n = min(vox, data.shape[-1])
     assert n > 128
     print(f"[large-gram] {name}: shape {data.shape} {data.dtype} patch {ps} overlap {po} n {n}",
           flush=True)
