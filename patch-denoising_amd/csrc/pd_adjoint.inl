@@ -1,5 +1,5 @@
 // Backward of optimal singular-value shrinkage with a per-patch threshold
-// scale lambda (included by pd_kernels.hip, inside its anonymous namespace).
+// scale lambda (included by pd_kernels.inl, inside its anonymous namespace).
 //
 // Forward, per selected patch (T-side: the centred Casorati matrix Xc is
 // M x T with T < M; Xc = U S V^T, n = T):

@@ -122,7 +122,7 @@ __global__ void window_std_kernel(const double* __restrict__ m, int64_t s0, int6
 }  // namespace
 
 namespace pdamd_detail {
-void set_error(const char* msg);  // pdamd_last_error's message (pd_kernels.hip)
+void set_error(const char* msg);  // pdamd_last_error's message (pd_api.hip)
 }
 
 namespace {

@@ -1,0 +1,18 @@
+// Pipelines for float32 data with a float32 eigen stage (forward, with the
+// gather recombination, and the optimal rule's backward).
+#define PD_RUN_GATHER 1
+#include "pd_kernels.inl"
+
+namespace pd {
+
+int run_f32_f32(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
+          StageTimer& tm) {
+  return run_chunks<float, float, float>(P, pl, ws, count, st, tm);
+}
+
+int backward_f32_f32(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, const void* gout,
+          const float* w4, void* gin, double* glam, hipStream_t st, StageTimer& tm) {
+  return run_backward<float, float, float>(P, pl, ws, count, gout, w4, gin, glam, st, tm);
+}
+
+}  // namespace pd

@@ -1,6 +1,6 @@
 // One-wavefront Householder tridiagonalisation of real Grams (n <= 128),
 // pd_tridiag_wave.hip.  Same inputs and outputs as the 4-wave register kernel
-// in pd_kernels.hip: the Gram (row-major n x n, both triangles) in each
+// in pd_kernels.inl: the Gram (row-major n x n, both triangles) in each
 // patch's H block on input; packed reflector rows in that block, d | e | e^2
 // in DE and tau in TAU on output.
 #pragma once
