@@ -167,6 +167,21 @@ typedef struct pdamd_problem {
                              weighted / average recombination, MP-PCA or
                              Hybrid; otherwise PDAMD_EUNSUPPORTED.          */
   int recombine_mode;
+
+  /* FP64 re-check of near-threshold decisions (no counterpart in the
+     reference).  Float32 data with an FP32 eigen stage, MP-PCA or Hybrid:
+     a patch where a comparison its rule makes -- an eigenvalue against
+     sigma^2 * threshold_scale^2, the Marchenko-Pastur iteration's scaled
+     mean against the eigenvalue spread, Hybrid's mean against the prior --
+     has its two sides within refine_rel of each other (relative to the
+     larger magnitude) is recomputed with an FP64 Gram, tridiagonalisation
+     and eigen stage (eigenvalues rounded to float32 for the rule, as the
+     reference's float32 spectrum), and that result is kept.  At most a fixed
+     number of patches per call (gather recombination) or per chunk is
+     re-checked (256 MiB of FP64 buffers, at most 1024 patches): the first in
+     patch order; the others keep the FP32 decision.  0 = off.             */
+  double refine_rel;
+  int32_t* refined_dev;  /* optional, per selected patch: 1 where re-checked */
 } pdamd_problem;
 
 enum { PDAMD_RECOMBINE_AUTO = 0, PDAMD_RECOMBINE_SCATTER = 1, PDAMD_RECOMBINE_GATHER = 2 };
@@ -187,7 +202,8 @@ enum {
   PDAMD_STAGE_ADJ_SYNTH = 8,    /* backward: dL/dX scatter-accumulate           */
   PDAMD_STAGE_RECOMBINE = 9,    /* gather recombination: every output element
                                    summed from its covering patches' factors */
-  PDAMD_NSTAGES = 10
+  PDAMD_STAGE_REFINE = 10,      /* FP64 re-check of near-threshold patches     */
+  PDAMD_NSTAGES = 11
 };
 
 /* Bytes of device workspace pdamd_denoise needs for this problem. */
@@ -245,6 +261,19 @@ int pdamd_noise_workspace_size(const int64_t* shape4, size_t* bytes);
 int pdamd_estimate_noise(const void* in_dev, int dtype, const int64_t* shape4, int nspatial,
                          int block_size, double* out_dev, void* workspace_dev,
                          size_t workspace_bytes, void* stream);
+
+/* Largest singular value of each of `batch` float64 matrices (rows x cols,
+   C order, consecutive in a_dev) into sv_dev (float64, batch): the square
+   root of the top eigenvalue of the min-side Gram, formed in FP64 (split
+   contraction summed in a fixed order: results repeat bit for bit),
+   tridiagonalised and bisected to full precision.  Replaces the
+   max(svd(draw, compute_uv=False)) of the Monte-Carlo thresholds in
+   space_time/lowrank.py:221-232 (NORDIC) and :446-456 (QUT); the draws stay
+   with the caller.  min(rows, cols) must be in 2 .. 2048, batch <= 65535. */
+int pdamd_top_sv_workspace_size(int batch, int64_t rows, int64_t cols, size_t* bytes);
+int pdamd_top_singular_values(const double* a_dev, int batch, int64_t rows, int64_t cols,
+                              double* sv_dev, void* workspace_dev, size_t workspace_bytes,
+                              void* stream);
 
 /* pdamd_denoise without any device allocation, for callers that cannot call
    hipMalloc (code under HIP graph capture, tight serving loops, harnesses

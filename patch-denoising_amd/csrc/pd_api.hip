@@ -130,6 +130,10 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
                   "rule and an uncapped workspace");
     pl->gather = elig && P->recombine_mode != PDAMD_RECOMBINE_SCATTER && chunk_cap == 0;
   }
+  // FP64 re-check of near-threshold decisions: float32 data whose eigen
+  // rule runs on FP32 eigenvalues
+  pl->refine = !bwd && P->refine_rel > 0 && P->dtype == PDAMD_F32 && pl->etype == 0 &&
+               (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID);
   // chunk: keep the per-chunk buffers under ~4 GiB (6 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
@@ -175,6 +179,34 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
     pl->off_GY = take(NPs * n * kGatherKM * sizeof(float));
     pl->off_GRF = take(NPs * (kGatherKM + 1) * pt * sizeof(float));
     pl->off_GW3 = take((size_t)pl->nvox * sizeof(float));
+  }
+  if (pl->refine) {
+    // per listed patch: FP64 Gram / reflectors, tridiagonal, tau,
+    // inverse-iteration scratch, eigenvectors (Gram and tridiagonal), rule
+    // outputs; at most 1024 patches and 256 MiB
+    const size_t per_r = (size_t)n * n * 8 + 3 * (size_t)n * 8 + (size_t)n * 8 +
+                         3 * (size_t)n * pl->kb * 8 + (size_t)pt * 4 +
+                         2 * (size_t)n * pl->kmax * 8 + (size_t)pl->kmax * 8 + 64 + 40 + 1024;
+    const int64_t cap = std::max<int64_t>(1, (int64_t)((256ull << 20) / per_r));
+    pl->rcap = (int)std::min<int64_t>({cap, (int64_t)1024, pl->chunk});
+    const size_t R = (size_t)pl->rcap;
+    pl->off_RFLAG = take((size_t)pl->npatch);
+    pl->off_RCNT = take(2 * sizeof(int));
+    pl->off_RSLOT = take(R * sizeof(int32_t));
+    pl->off_RIDX = take(R * sizeof(int32_t));
+    pl->off_RH = take(R * n * n * 8);
+    pl->off_RDE = take(R * 3 * n * 8);
+    pl->off_RTAU = take(R * n * 8);
+    pl->off_RSCR = take(R * 3 * n * pl->kb * 8);
+    pl->off_RMEAN = take(R * pt * 4);
+    pl->off_RY = take(R * n * pl->kmax * 8);
+    pl->off_RZ = take(R * n * pl->kmax * 8);
+    pl->off_RWC = take(R * pl->kmax * 8);
+    pl->off_RNC = take(R * sizeof(int));
+    pl->off_RTH = take(R * sizeof(double));
+    pl->off_RRK = take(R * sizeof(int32_t));
+    pl->off_RNZ = take(R * sizeof(float));
+    pl->off_REG = take(R * 10 * sizeof(float));
   }
   if (bwd) {
     pl->off_GH = take((size_t)pl->nvox * g.T * pl->ebytes);

@@ -87,6 +87,14 @@ struct Plan {
   // gather recombination only (per call, indexed by selected-patch slot)
   size_t off_GMAP = 0, off_GNC = 0, off_GTH = 0, off_GMEAN = 0, off_GY = 0, off_GRF = 0,
          off_GW3 = 0;
+  // FP64 re-check of near-threshold decisions (float32 data, FP32 eigen
+  // stage, MP-PCA / Hybrid): per-patch flags, the list of at most rcap
+  // flagged patches, and that list's FP64 Gram / Householder / eigen buffers
+  bool refine = false;
+  int rcap = 0;
+  size_t off_RFLAG = 0, off_RCNT = 0, off_RSLOT = 0, off_RIDX = 0, off_RH = 0, off_RDE = 0,
+         off_RTAU = 0, off_RSCR = 0, off_RMEAN = 0, off_RY = 0, off_RZ = 0, off_RWC = 0,
+         off_RNC = 0, off_RTH = 0, off_RRK = 0, off_RNZ = 0, off_REG = 0;
   // backward only
   size_t off_P1 = 0, off_P2 = 0, off_PE = 0, off_GS = 0, off_S1 = 0, off_UV = 0, off_THB = 0,
          off_GH = 0;
@@ -110,6 +118,17 @@ int run_c64_c128(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipS
                  StageTimer& tm);
 int run_c128(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
              StageTimer& tm);
+
+// FP64 re-check (pd_run_f32e64.hip) of the listed near-threshold patches of
+// a float32 problem: the linear patch indices at ws + off_RIDX, their count
+// (<= rcap) at ws + off_RCNT; Gram, tridiagonalisation and eigen rule in
+// FP64 into the off_R* buffers (eigenvectors n x kmax per list entry).
+int refine_f32_f64(pdamd_problem* P, const Plan& pl, char* ws, hipStream_t st);
+
+// Householder tridiagonalisation (pd_run_f64.hip) of `batch` symmetric
+// float64 n x n matrices at H (both triangles; overwritten by reflectors):
+// d | e | e^2 rows of 3n into DE, tau into TAU.
+int tridiag_f64(double* H, double* DE, double* TAU, int batch, int n, hipStream_t st);
 
 // Backward of the optimal rule (real data): float32 / float32, float32 /
 // float64, float64.

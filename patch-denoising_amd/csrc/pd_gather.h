@@ -45,6 +45,12 @@ struct GatherArgs {
   int mask_kind;
   float* out;            // output volume
   float* wout;           // float32 weights, volume shaped
+  // factor kernel over a list instead of a chunk: blocks q < *list_cnt
+  // take slot list[q] (component weights 1, as for the eigen rules); the
+  // launch has list_cap blocks
+  const int32_t* list = nullptr;
+  const int* list_cnt = nullptr;
+  int list_cap = 0;
 };
 
 // Each enqueues on `st` and returns 0, or -1 if the launch failed.

@@ -37,8 +37,9 @@ __device__ __forceinline__ void covering(int c, int p, int s, int gcount, int& l
 // independent data loads are in flight per row).
 template <int NCH>
 __global__ __launch_bounds__(kWave) void gather_factor_kernel(GatherArgs a) {
-  const int p = blockIdx.x;  // chunk-local patch
-  const int slot = a.slot0 + p;
+  const int p = blockIdx.x;  // chunk-local patch, or list entry
+  if (a.list && p >= *a.list_cnt) return;
+  const int slot = a.list ? a.list[p] : a.slot0 + p;
   const int lane = threadIdx.x;
   const Geom& g = a.g;
   const int M = g.M, T = g.pt;
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(kWave) void gather_factor_kernel(GatherArgs a) {
   float scale[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k)
-    scale[k] = k < nc ? (float)(theta * a.WC[(size_t)p * KM + k]) : 0.f;
+    scale[k] = k < nc ? (float)(theta * (a.list ? 1.0 : a.WC[(size_t)p * KM + k])) : 0.f;
   // patch-relative row offsets, rows lane and lane + 64 (M <= 128)
   auto roff = [&](int r) {
     const int i2 = r % g.p2, q = r / g.p2;
@@ -342,11 +343,12 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
 namespace pd {
 
 int launch_gather_factors(const GatherArgs& a, hipStream_t st) {
-  if (a.np < 1) return 0;
+  const int nb = a.list ? a.list_cap : a.np;
+  if (nb < 1) return 0;
   const int nch = (a.g.pt + kWave - 1) / kWave;
-  if (nch <= 2) gather_factor_kernel<2><<<a.np, kWave, 0, st>>>(a);
-  else if (nch <= 3) gather_factor_kernel<3><<<a.np, kWave, 0, st>>>(a);
-  else gather_factor_kernel<5><<<a.np, kWave, 0, st>>>(a);
+  if (nch <= 2) gather_factor_kernel<2><<<nb, kWave, 0, st>>>(a);
+  else if (nch <= 3) gather_factor_kernel<3><<<nb, kWave, 0, st>>>(a);
+  else gather_factor_kernel<5><<<nb, kWave, 0, st>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -368,6 +368,7 @@ struct GramArgs {
   void* TAU;           // np x n
   void* MEAN;          // np x pt per-frame patch means (input type), reused by the synthesis
   void* MEANW;         // M-side: np x pt per-frame means out (gather recombination), or null
+  const int* np_dev;   // device patch count (<= np, read by the kernel), or null: np
 };
 
 // Staging geometry: M-side stages kTC frames x (all rows); T-side stages kRC
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   __bf16* Sh = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;  // Sh[i][k]
   __bf16* Sl = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;
 
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(kNT) void tridiag_kernel(GramArgs a) {
   R* dd = lds_carve<R>(cur, 128);
   R* ee = lds_carve<R>(cur, 128);
   E* tt = lds_carve<E>(cur, 128);
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     E* Hp = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     E areg[RJ][2];
 #pragma unroll
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   };
   const int nb = (n + kGBT - 1) / kGBT;
   const int ntile = nb * (nb + 1) / 2;
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
@@ -1010,7 +1011,7 @@ __global__ __launch_bounds__(kG1T) void gram_big1_kernel(const IN* __restrict__ 
     bi[b] = r;
     bj[b] = blk - r * (r + 1) / 2;
   }
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
@@ -1205,7 +1206,7 @@ __global__ __launch_bounds__(NT) void tridiag_big_kernel(GramArgs a, int k0) {
   E* vg = Bp;
   E* wg = Bp + n;
   const E zero = E(R(0));
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     E* A = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     R* DEp = reinterpret_cast<R*>(a.DE) + (size_t)p * 3 * n;
     E* TAUp = reinterpret_cast<E*>(a.TAU) + (size_t)p * n;
@@ -1451,6 +1452,11 @@ struct EigArgs {
   float* NOISE;      // np or null
   float* EIG;        // np x 10 or null
   int zero_fill;     // 1: eigenvector columns ncomp .. kmax-1 of Y are set to zero
+  const int* np_dev; // device patch count (<= np, read by the kernel), or null: np
+  // FP64 re-check of near-threshold decisions (MP-PCA / Hybrid): NEAR[p] = 1
+  // when a comparison of the rule has its sides within refine_rel relative
+  double refine_rel;
+  uint8_t* NEAR;     // np flags (written when non-null)
 };
 
 template <class R> struct Lim;
@@ -1964,7 +1970,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
   __shared__ int sh_i[4];
   __shared__ double sh_d[4];
 
-  for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
+  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     {  // the tridiagonal into LDS: every recurrence below reads it
       const R* src = reinterpret_cast<const R*>(a.DE) + (size_t)p * 3 * n;
       for (int i = lane; i < n; i += kWave) {
@@ -2026,32 +2032,50 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         }
         int k = 0;
         RT var;
+        // a decision whose two sides lie within refine_rel of each other
+        // (relative to the larger) is re-checked in FP64 when NEAR is set
+        const double rr = a.refine_rel;
+        bool near = false;
+        auto close = [&](double x, double y) {
+          near = near || fabs(x - y) <= rr * fmax(fabs(x), fabs(y));
+        };
         if (a.method == PDAMD_MPPCA) {
           // lowrank.py:45-54
           double mv = (double)np_mean10(ev, 10) * (4.0 * sqrt((double)(10 - k + 1) / a.g.M));
+          close(mv, (double)(ev[9 - k] - ev[0]));
           while (k < 10 && mv < (double)(ev[9 - k] - ev[0])) {
             ++k;
             mv = (double)np_mean10(ev, 10 - k) * (4.0 * sqrt((double)(a.g.pt - k + 1) / a.g.M));
+            if (k < 10) close(mv, (double)(ev[9 - k] - ev[0]));
           }
           var = np_mean10(ev, 10 - k);
           int r = 0;
           if (a.scale_f64) {
             const double thr = (double)var * a.ts2;
-            for (int q = 0; q < 10; ++q) r += ((double)ev[q] > thr);
+            for (int q = 0; q < 10; ++q) {
+              r += ((double)ev[q] > thr);
+              close((double)ev[q], thr);
+            }
           } else {
             const RT thr = var * (RT)a.ts2;
-            for (int q = 0; q < 10; ++q) r += (ev[q] > thr);
+            for (int q = 0; q < 10; ++q) {
+              r += (ev[q] > thr);
+              close((double)ev[q], (double)thr);
+            }
           }
           rank = r;
         } else {
           // lowrank.py:108-112
           var = np_mean10(ev, 10);
+          close((double)var, prior);
           while ((double)var > prior && k < 10 - 2) {
             ++k;
             var = np_mean10(ev, 10 - k);
+            close((double)var, prior);
           }
           rank = k;
         }
+        if (a.NEAR) a.NEAR[p] = near ? 1 : 0;
         noise = (float)var;
         ncomp = min(rank, nt);
         for (int c = 0; c < ncomp; ++c) {
@@ -2986,6 +3010,7 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       ta.TAU = ga.TAU;
       ta.np = ga.np;
       ta.n = pl.n;
+      ta.np_dev = ga.np_dev;
       if (launch_tridiag_wave(ta, F64, st))
         return fail(PDAMD_EHIP, "tridiagonalisation launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
@@ -3104,6 +3129,9 @@ EigArgs eig_args(const pdamd_problem* P, const Plan& pl, char* ws, const GramArg
   ea.NOISE = nullptr;
   ea.EIG = nullptr;
   ea.zero_fill = 0;
+  ea.np_dev = nullptr;
+  ea.refine_rel = 0.0;
+  ea.NEAR = nullptr;
   return ea;
 }
 
@@ -3119,7 +3147,130 @@ GramArgs gram_args(const pdamd_problem* P, const Plan& pl, char* ws, const int32
   ga.TAU = ws + pl.off_TAU;
   ga.MEAN = ws + pl.off_MEAN;
   ga.MEANW = nullptr;
+  ga.np_dev = nullptr;
   return ga;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 re-check of near-threshold rule decisions (float32 data, FP32 eigen
+// stage, MP-PCA / Hybrid).  The eigen kernel flags a patch when a comparison
+// of its rule has both sides within refine_rel of each other; the flagged
+// patches are listed in increasing slot order (at most rcap), recomputed with
+// an FP64 Gram, tridiagonalisation and eigen stage (refine_f32_f64), and the
+// FP64 result replaces the FP32 one in the buffers the synthesis reads.
+// ---------------------------------------------------------------------------
+
+// One 1024-thread workgroup: slots[q] / ridx[q] = the q-th flagged slot and
+// its linear patch index; cnt[0] = listed (<= cap), cnt[1] = flagged.
+__global__ __launch_bounds__(1024) void refine_compact_kernel(
+    const uint8_t* __restrict__ flags, int nflags, const int32_t* __restrict__ idx, int cap,
+    int* __restrict__ cnt, int32_t* __restrict__ slots, int32_t* __restrict__ ridx) {
+  __shared__ int wsum[16];
+  __shared__ int base_s;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  if (tid == 0) base_s = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nflags; b0 += 1024) {
+    const int i = b0 + tid;
+    const bool f = i < nflags && flags[i] != 0;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = base_s;
+    for (int j = 0; j < w; ++j) off += wsum[j];
+    if (f) {
+      const int q = off + __popcll(m & ((1ull << lane) - 1ull));
+      if (q < cap) {
+        slots[q] = i;
+        ridx[q] = idx[i];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int t = 0;
+      for (int j = 0; j < 1024 / kWave; ++j) t += wsum[j];
+      base_s += t;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cnt[0] = min(base_s, cap);
+    cnt[1] = base_s;
+  }
+}
+
+struct RefineStore {
+  const int* cnt;          // [0]: listed patches
+  const int32_t* slots;    // slot of each listed patch in the destination arrays
+  int n, kmax_src, kmax_dst;
+  const double* Ysrc;      // per list entry: n x kmax_src FP64 eigenvectors
+  float* Ydst;             // per slot: n x kmax_dst
+  const int* NCsrc;
+  int* NCdst;
+  const double* THsrc;
+  double* THdst;
+  const double* WCsrc;
+  double* WCdst;           // or null
+  const int32_t* RKsrc;
+  int32_t* RKdst;          // or null (and the following)
+  const float* NZsrc;
+  float* NZdst;
+  const float* EGsrc;
+  float* EGdst;
+  int32_t* REFINED;        // 1 at each re-checked slot, or null
+};
+
+// One workgroup per list entry (blocks past the listed count exit).
+__global__ __launch_bounds__(kNT) void refine_store_kernel(RefineStore a) {
+  const int q = blockIdx.x;
+  if (q >= a.cnt[0]) return;
+  const int s = a.slots[q];
+  const int tid = threadIdx.x;
+  const int nc = a.NCsrc[q];
+  const double* __restrict__ ys = a.Ysrc + (size_t)q * a.n * a.kmax_src;
+  float* __restrict__ yd = a.Ydst + (size_t)s * a.n * a.kmax_dst;
+  for (int e = tid; e < a.n * a.kmax_dst; e += kNT) {
+    const int i = e / a.kmax_dst, c = e - i * a.kmax_dst;
+    yd[e] = c < nc ? (float)ys[(size_t)i * a.kmax_src + c] : 0.f;
+  }
+  if (a.WCdst && tid < a.kmax_dst)
+    a.WCdst[(size_t)s * a.kmax_dst + tid] = tid < nc ? a.WCsrc[(size_t)q * a.kmax_src + tid] : 0.0;
+  if (a.EGdst && tid < 10) a.EGdst[(size_t)s * 10 + tid] = a.EGsrc[(size_t)q * 10 + tid];
+  if (tid == 0) {
+    a.NCdst[s] = nc;
+    a.THdst[s] = a.THsrc[q];
+    if (a.RKdst) a.RKdst[s] = a.RKsrc[q];
+    if (a.NZdst) a.NZdst[s] = a.NZsrc[q];
+    if (a.REFINED) a.REFINED[s] = 1;
+  }
+}
+
+// List the flagged patches (flags / idx over `nflags` slots), re-check them
+// in FP64 and store the results through `sto` (its slot-indexed pointers and
+// kmax_dst filled in by the caller).
+int run_refine(pdamd_problem* P, const Plan& pl, char* ws, const uint8_t* flags, int nflags,
+               const int32_t* idx, RefineStore sto, hipStream_t st) {
+  int* cnt = reinterpret_cast<int*>(ws + pl.off_RCNT);
+  int32_t* slots = reinterpret_cast<int32_t*>(ws + pl.off_RSLOT);
+  refine_compact_kernel<<<1, 1024, 0, st>>>(flags, nflags, idx, pl.rcap, cnt, slots,
+                                            reinterpret_cast<int32_t*>(ws + pl.off_RIDX));
+  HIP_TRY(hipGetLastError());
+  int rc = refine_f32_f64(P, pl, ws, st);
+  if (rc) return rc;
+  sto.cnt = cnt;
+  sto.slots = slots;
+  sto.n = pl.n;
+  sto.kmax_src = pl.kmax;
+  sto.Ysrc = reinterpret_cast<const double*>(ws + pl.off_RY);
+  sto.NCsrc = reinterpret_cast<const int*>(ws + pl.off_RNC);
+  sto.THsrc = reinterpret_cast<const double*>(ws + pl.off_RTH);
+  sto.WCsrc = reinterpret_cast<const double*>(ws + pl.off_RWC);
+  sto.RKsrc = reinterpret_cast<const int32_t*>(ws + pl.off_RRK);
+  sto.NZsrc = reinterpret_cast<const float*>(ws + pl.off_RNZ);
+  sto.EGsrc = reinterpret_cast<const float*>(ws + pl.off_REG);
+  refine_store_kernel<<<pl.rcap, kNT, 0, st>>>(sto);
+  HIP_TRY(hipGetLastError());
+  return PDAMD_OK;
 }
 
 #include "pd_adjoint.inl"
@@ -3134,11 +3285,16 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
 template <class IN, class E, class RT>
 int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
                StageTimer& tm) {
+  if (P->refined_dev && count > 0) {
+    const int rz = zero_async(P->refined_dev, (size_t)count * sizeof(int32_t), st);
+    if (rz) return rz;
+  }
   if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value)
     if (pl.gather) return run_chunks_gather(P, pl, ws, count, st, tm);
   const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
   double* wacc = reinterpret_cast<double*>(ws + pl.off_w3);
   const bool w3d = pl.g.gt == 1;
+  uint8_t* rflags = reinterpret_cast<uint8_t*>(ws + pl.off_RFLAG);
   for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
     const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
     GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
@@ -3155,10 +3311,32 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
     ea.RANK = P->rank_dev ? P->rank_dev + c0 : nullptr;
     ea.NOISE = P->noise_dev ? P->noise_dev + c0 : nullptr;
     ea.EIG = P->eig_dev ? P->eig_dev + c0 * 10 : nullptr;
+    if (pl.refine) {
+      ea.refine_rel = P->refine_rel;
+      ea.NEAR = rflags;
+    }
     tm.begin(PDAMD_STAGE_EIG);
     rc = launch_eig<E, RT>(pl, ea, st);
     tm.end();
     if (rc) return rc;
+    if constexpr (std::is_same<IN, float>::value && std::is_same<E, float>::value) {
+      if (pl.refine) {  // the chunk's near-threshold patches, in FP64
+        RefineStore sto{};
+        sto.kmax_dst = pl.kmax;
+        sto.Ydst = reinterpret_cast<float*>(ea.Y);
+        sto.NCdst = ea.NCOMP;
+        sto.THdst = ea.THETA;
+        sto.WCdst = ea.WC;
+        sto.RKdst = ea.RANK;
+        sto.NZdst = ea.NOISE;
+        sto.EGdst = ea.EIG;
+        sto.REFINED = P->refined_dev ? P->refined_dev + c0 : nullptr;
+        tm.begin(PDAMD_STAGE_REFINE);
+        rc = run_refine(P, pl, ws, rflags, np, idx_all + c0, sto, st);
+        tm.end();
+        if (rc) return rc;
+      }
+    }
     SynthArgs sa;
     sa.g = pl.g;
     sa.idx = ga.idx;
@@ -3240,6 +3418,10 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
     ea.RANK = P->rank_dev ? P->rank_dev + c0 : nullptr;
     ea.NOISE = P->noise_dev ? P->noise_dev + c0 : nullptr;
     ea.EIG = P->eig_dev ? P->eig_dev + c0 * 10 : nullptr;
+    if (pl.refine) {
+      ea.refine_rel = P->refine_rel;
+      ea.NEAR = reinterpret_cast<uint8_t*>(ws + pl.off_RFLAG) + c0;
+    }
     tm.begin(PDAMD_STAGE_EIG);
     rc = launch_eig<float, float>(pl, ea, st);
     tm.end();
@@ -3251,6 +3433,31 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
     rc = launch_gather_factors(fa, st);
     tm.end();
     if (rc) return fail(PDAMD_EHIP, "factor kernel launch failed");
+  }
+  if (pl.refine && count > 0) {
+    // the call's near-threshold patches in FP64, then their factors again
+    RefineStore sto{};
+    sto.kmax_dst = kGatherKM;
+    sto.Ydst = reinterpret_cast<float*>(ws + pl.off_GY);
+    sto.NCdst = reinterpret_cast<int*>(ws + pl.off_GNC);
+    sto.THdst = reinterpret_cast<double*>(ws + pl.off_GTH);
+    sto.WCdst = nullptr;
+    sto.RKdst = P->rank_dev;
+    sto.NZdst = P->noise_dev;
+    sto.EGdst = P->eig_dev;
+    sto.REFINED = P->refined_dev;
+    tm.begin(PDAMD_STAGE_REFINE);
+    int rc = run_refine(P, pl, ws, reinterpret_cast<const uint8_t*>(ws + pl.off_RFLAG),
+                        (int)count, idx_all, sto, st);
+    if (!rc) {
+      GatherArgs fl = fa;
+      fl.list = reinterpret_cast<const int32_t*>(ws + pl.off_RSLOT);
+      fl.list_cnt = reinterpret_cast<const int*>(ws + pl.off_RCNT);
+      fl.list_cap = pl.rcap;
+      if (launch_gather_factors(fl, st)) rc = fail(PDAMD_EHIP, "factor kernel launch failed");
+    }
+    tm.end();
+    if (rc) return rc;
   }
   tm.begin(PDAMD_STAGE_FINALIZE);
   int rc = launch_gather_map(fa, pl.npatch, count, st);

@@ -16,6 +16,7 @@ struct TridiagWaveArgs {
   void* TAU;  // np x n
   int np;     // patches
   int n;      // Gram size
+  const int* np_dev = nullptr;  // device patch count (<= np): blocks past it exit
 };
 
 // Blocks of 8 per side for a Gram of size n (0: not handled here: n > 128,

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   // one patch per workgroup: no patch loop, so no per-lane load offsets are
   // hoisted and kept live across patches
   const int p = blockIdx.x;
+  if (args.np_dev && p >= *args.np_dev) return;
   R* Hp = reinterpret_cast<R*>(args.H) + (size_t)p * n * n;
   R L[NT];
 #pragma unroll

@@ -71,11 +71,17 @@ class Problem(ctypes.Structure):
         ("qut_tau_dev", ctypes.c_void_p),
         ("launch_per_cu", ctypes.c_int * 4),
         ("recombine_mode", ctypes.c_int),
+        ("refine_rel", ctypes.c_double),
+        ("refined_dev", ctypes.c_void_p),
     ]
 
 
+# Relative gap below which an MP-PCA / Hybrid decision taken on FP32
+# eigenvalues is re-checked with an FP64 eigen stage (pdamd_problem.refine_rel).
+REFINE_REL = 1e-6
+
 STAGES = ("select", "gram", "trieig", "synth", "finalize", "tridiag", "adj_cross", "adj_spectral",
-          "adj_synth", "recombine")
+          "adj_synth", "recombine", "refine")
 
 
 EXPORTS = (
@@ -87,6 +93,8 @@ EXPORTS = (
     "pdamd_backward_workspace_size",
     "pdamd_optimal_backward",
     "pdamd_noise_workspace_size",
+    "pdamd_top_sv_workspace_size",
+    "pdamd_top_singular_values",
     "pdamd_estimate_noise",
     "pdamd_noalloc_scratch_bytes",
     "pdamd_denoise_noalloc",
@@ -143,6 +151,13 @@ def load_library():
                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_size_t, ctypes.c_void_p]
     lib.pdamd_estimate_noise.restype = ctypes.c_int
+    lib.pdamd_top_sv_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_size_t)]
+    lib.pdamd_top_sv_workspace_size.restype = ctypes.c_int
+    lib.pdamd_top_singular_values.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                              ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_size_t, ctypes.c_void_p]
+    lib.pdamd_top_singular_values.restype = ctypes.c_int
     lib.pdamd_noalloc_scratch_bytes.argtypes = [ctypes.POINTER(ctypes.c_size_t)]
     lib.pdamd_noalloc_scratch_bytes.restype = ctypes.c_int
     lib.pdamd_denoise_noalloc.argtypes = [P, ctypes.c_void_p]
@@ -199,6 +214,7 @@ def make_problem(shape4, patch4, step4, dtype, method, recombination="weighted",
     p.mask_kind = MASK_NONE
     p.mask_threshold = 50.0
     p.tau0_log = float("nan")
+    p.refine_rel = REFINE_REL
     return p
 
 
@@ -256,11 +272,13 @@ class DeviceRun:
                 rank=torch.empty(cap, dtype=torch.int32, device=dev),
                 noise=torch.empty(cap, dtype=torch.float32, device=dev),
                 eig=torch.empty(cap * 10, dtype=torch.float32, device=dev),
+                refined=torch.empty(cap, dtype=torch.int32, device=dev),
             )
             prob.patch_index_dev = self.info["index"].data_ptr()
             prob.rank_dev = self.info["rank"].data_ptr()
             prob.noise_dev = self.info["noise"].data_ptr()
             prob.eig_dev = self.info["eig"].data_ptr()
+            prob.refined_dev = self.info["refined"].data_ptr()
         self.ws_bytes, self.ws = 0, None
         if workspace:
             self.ws_bytes = workspace_bytes(prob)
@@ -289,6 +307,7 @@ class DeviceRun:
             rank=i["rank"][:n].cpu().numpy(),
             noise=i["noise"][:n].cpu().numpy(),
             eig=i["eig"][: n * 10].cpu().numpy().reshape(n, 10),
+            refined=i["refined"][:n].cpu().numpy(),
         )
 
 
@@ -393,3 +412,32 @@ def estimate_noise(data, block_size):
                                     ws.data_ptr(), int(nb.value),
                                     torch.cuda.current_stream().cuda_stream))
     return out.reshape(shape[:-1])
+
+
+def top_singular_values(a):
+    """Largest singular value of a float64 matrix, or of each matrix of a
+    stack (``a``: (rows, cols) or (batch, rows, cols), NumPy or CUDA tensor),
+    on the GPU (``pdamd_top_singular_values``).  Returns a NumPy float64 array
+    of shape (batch,) (or a 0-d array for one matrix)."""
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    if isinstance(a, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    else:
+        x = a.to(device=dev, dtype=torch.float64).contiguous()
+    one = x.dim() == 2
+    if one:
+        x = x.unsqueeze(0)
+    if x.dim() != 3:
+        raise ValueError("top_singular_values takes a matrix or a stack of matrices")
+    batch, rows, cols = (int(v) for v in x.shape)
+    lib = load_library()
+    nb = ctypes.c_size_t(0)
+    _check(lib.pdamd_top_sv_workspace_size(batch, rows, cols, ctypes.byref(nb)))
+    ws = torch.empty(max(int(nb.value), 8), dtype=torch.uint8, device=dev)
+    sv = torch.empty(batch, dtype=torch.float64, device=dev)
+    _check(lib.pdamd_top_singular_values(x.data_ptr(), batch, rows, cols, sv.data_ptr(),
+                                         ws.data_ptr(), int(nb.value),
+                                         torch.cuda.current_stream().cuda_stream))
+    out = sv.cpu().numpy()
+    return out[0] if one else out

@@ -10,7 +10,7 @@ Marchenko-Pastur median) and the prior variance map.
 from types import MappingProxyType
 
 import numpy as np
-from scipy.linalg import eigvalsh, svd
+from scipy.linalg import svd
 
 from .. import _native as N
 from .base import BaseSpaceTimeDenoiser, PatchedArray, spatial_variance_map
@@ -118,10 +118,11 @@ def nordic_threshold(n_rows, n_frames, n_iter):
     ``np.random`` stream (reference: lowrank.py:221-232).
 
     Small draws use the same ``scipy.linalg.svd`` call as the reference; large
-    ones take the largest eigenvalue of the float64 Gram ``a^T a`` (same value
-    to ~1e-15 relative), with the Gram formed on the GPU (a plain FP64 library
-    GEMM) when one is present.  The draws themselves stay on the host: they
-    must be the global stream's own legacy Gaussian sequence.
+    ones (NORDIC's draws include the frames in the row count: 266,200 x 200
+    for 11^3-voxel patches of 200 frames) go to the library's
+    ``pdamd_top_singular_values`` (FP64 Gram, Householder, bisection: the
+    same value to ~1e-15 relative).  The draws themselves stay on the host:
+    they must be the global stream's own legacy Gaussian sequence.
     """
     total = 0.0
     for _ in range(n_iter):
@@ -129,21 +130,8 @@ def nordic_threshold(n_rows, n_frames, n_iter):
         if a.size <= 4_000_000:
             total = total + max(svd(a, compute_uv=False))
         else:
-            g = _gram_f64(a)
-            total = total + np.sqrt(eigvalsh(g, subset_by_index=(n_frames - 1, n_frames - 1))[0])
+            total = total + float(N.top_singular_values(a))
     return np.float64(total) / n_iter
-
-
-def _gram_f64(a):
-    """a^T a in float64, on the GPU when available."""
-    try:
-        import torch
-    except ImportError:  # pragma: no cover - torch is part of the runtime
-        torch = None
-    if torch is not None and torch.cuda.is_available():
-        d = torch.from_numpy(a).cuda()
-        return (d.T @ d).cpu().numpy()
-    return a.T @ a
 
 
 class NordicDenoiser(RawSVDDenoiser):
@@ -228,19 +216,11 @@ def _qut_max_svals(n_rows, n_frames, stdest, nbsim):
     n_frames) * stdest`` from the global legacy ``np.random`` stream, in the
     reference's order (lowrank.py:446-456: one ``randn(*patch.shape)`` per
     draw; a single ``randn(nbsim, n_rows, n_frames)`` yields the same
-    numbers).  The values come from the float64 min-side Gram's top
-    eigenvalue, on the GPU when present (same value as the SVD to ~1e-15)."""
+    numbers).  The values come from the library's
+    ``pdamd_top_singular_values`` over the whole stack (the float64 min-side
+    Gram's top eigenvalue: the SVD's value to ~1e-15)."""
     a = np.random.randn(nbsim, n_rows, n_frames) * stdest
-    try:
-        import torch
-    except ImportError:  # pragma: no cover - torch is part of the runtime
-        torch = None
-    if torch is not None and torch.cuda.is_available():
-        d = torch.from_numpy(a).cuda()
-        g = d.transpose(1, 2).contiguous() @ d if n_frames <= n_rows else d @ d.transpose(1, 2).contiguous()
-        top = torch.linalg.eigvalsh(g)[:, -1]
-        return torch.sqrt(torch.clamp(top, min=0)).cpu().numpy()
-    return np.array([max(svd(x, compute_uv=False)) for x in a])
+    return N.top_singular_values(a)
 
 
 class AdaptiveDenoiser(BaseSpaceTimeDenoiser):
