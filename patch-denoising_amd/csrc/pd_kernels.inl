@@ -395,6 +395,50 @@ template <class R> __device__ __forceinline__ cpx<R> div_n(cpx<R> a, int n) {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kBfW = 32 + 8;  // bf16 staging row: SC frames + pad (80 B, 16-B aligned)
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Lower-triangle 16 x 16 tiles (I >= J) of a padded 128 x 128 Gram, nine per
+// wave (36 in all): wave 0 the leading 4 x 4 block triangle without (3, 3),
+// wave 1 the trailing one without (7, 7), waves 2 / 3 block rows 4..7 x block
+// columns {0, 1} / {2, 3} plus (3, 3) / (7, 7).
+template <int W> struct Tri16;
+template <> struct Tri16<0> {
+  static constexpr int I[9] = {0, 1, 1, 2, 2, 2, 3, 3, 3}, J[9] = {0, 0, 1, 0, 1, 2, 0, 1, 2};
+};
+template <> struct Tri16<1> {
+  static constexpr int I[9] = {4, 5, 5, 6, 6, 6, 7, 7, 7}, J[9] = {4, 4, 5, 4, 5, 6, 4, 5, 6};
+};
+template <> struct Tri16<2> {
+  static constexpr int I[9] = {4, 4, 5, 5, 6, 6, 7, 7, 3}, J[9] = {0, 1, 0, 1, 0, 1, 0, 1, 3};
+};
+template <> struct Tri16<3> {
+  static constexpr int I[9] = {4, 4, 5, 5, 6, 6, 7, 7, 7}, J[9] = {2, 3, 2, 3, 2, 3, 2, 3, 7};
+};
+template <int W> constexpr bool tri16_uses(int b) {
+  for (int t = 0; t < 9; ++t)
+    if (Tri16<W>::I[t] == b || Tri16<W>::J[t] == b) return true;
+  return false;
+}
+
+// acc[t] += S[:, 16 I_t ..][k] S[:, 16 J_t ..][k]^T over reduction indices
+// k < kcount (a multiple of 4 may run past kcount: those staged rows are
+// zero) on mfma_f32_16x16x4f32.  S is staged [k][i] with row length ldw.
+// Each 16-index block the wave needs is read once per step of 4.
+template <int W>
+__device__ __forceinline__ void tri16_accum(const float* __restrict__ S, int ldw, int kcount,
+                                            int lane, floatx4 (&acc)[9]) {
+  for (int k = 0; k < kcount; k += 4) {
+    const float* row = S + (k + (lane >> 4)) * ldw + (lane & 15);
+    float v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = tri16_uses<W>(b) ? row[16 * b] : 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[Tri16<W>::I[t]], v[Tri16<W>::J[t]], acc[t],
+                                                    0, 0, 0);
+  }
+}
+
 template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
@@ -417,6 +461,8 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   // lo = bf16(x - hi); G += hi hi^T + hi lo^T + lo hi^T on MFMA 32x32x16 bf16
   // with FP32 accumulation (the lo lo^T term, ~2^-16 relative, is dropped).
   constexpr bool BFM = BF && MF && SC == 32;  // M-side: planes [row][frame]; T-side: [frame][row]
+  // FP32 (no split): lower-triangle 16 x 16 tiles on MFMA 16x16x4 (Tri16)
+  constexpr bool M16 = MF && !BF;
   const int wv = tid / kWave, lane = tid % kWave;
   const int bi = wv >> 1, bj = wv & 1, lhi = lane >> 5, llo = lane & 31;
 
@@ -438,7 +484,13 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 
     E acc[MF ? 1 : TILE][MF ? 1 : TILE];
     floatx16 mc[2][2];
-    if constexpr (MF) {
+    floatx4 m16[M16 ? 9 : 1];
+    if constexpr (M16) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m16[t][q] = 0.0f;
+    } else if constexpr (MF) {
 #pragma unroll
       for (int a2 = 0; a2 < 2; ++a2)
 #pragma unroll
@@ -473,15 +525,13 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
               mc[a2][b2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la[a2], hb[b2], mc[a2][b2], 0, 0, 0);
             }
         }
-      } else if constexpr (MF) {
-        for (int k = 0; k < kcount; k += 2) {
-          const float* row = reinterpret_cast<const float*>(Sk) + (k + lhi) * LDS_W;
-          const float a0 = row[64 * bi + llo], a1 = row[64 * bi + 32 + llo];
-          const float b0 = row[64 * bj + llo], b1 = row[64 * bj + 32 + llo];
-          mc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, mc[0][0], 0, 0, 0);
-          mc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, mc[0][1], 0, 0, 0);
-          mc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, mc[1][0], 0, 0, 0);
-          mc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, mc[1][1], 0, 0, 0);
+      } else if constexpr (M16) {
+        const float* S = reinterpret_cast<const float*>(Sk);
+        switch (wv) {
+          case 0: tri16_accum<0>(S, LDS_W, kcount, lane, m16); break;
+          case 1: tri16_accum<1>(S, LDS_W, kcount, lane, m16); break;
+          case 2: tri16_accum<2>(S, LDS_W, kcount, lane, m16); break;
+          default: tri16_accum<3>(S, LDS_W, kcount, lane, m16); break;
         }
       }
     };
@@ -624,7 +674,31 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
     // ---- Gram -> this patch's H block (row-major, optionally / (M-1) as the
     //      reference's eigen path)
     E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
-    if constexpr (MF) {
+    if constexpr (M16) {
+      // 16x16 MFMA output: accumulator q of lane l is C[4(l/16) + q][l%16];
+      // off-diagonal tiles are written to both triangles
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        int I, J;
+        switch (wv) {
+          case 0: I = Tri16<0>::I[t]; J = Tri16<0>::J[t]; break;
+          case 1: I = Tri16<1>::I[t]; J = Tri16<1>::J[t]; break;
+          case 2: I = Tri16<2>::I[t]; J = Tri16<2>::J[t]; break;
+          default: I = Tri16<3>::I[t]; J = Tri16<3>::J[t]; break;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * I + 4 * (lane >> 4) + q, c = 16 * J + (lane & 15);
+          if (r < n && c < n) {
+            float v = m16[t][q];
+            if (a.scale) v = div_n(v, M - 1);
+            float* Gf = reinterpret_cast<float*>(G);
+            Gf[(size_t)r * n + c] = v;
+            if (I != J) Gf[(size_t)c * n + r] = v;
+          }
+        }
+      }
+    } else if constexpr (MF) {
       // 32x32 MFMA output: accumulator q of lane l is C[8(q/4) + 4(l/32) + q%4][l%32]
 #pragma unroll
       for (int a2 = 0; a2 < 2; ++a2)
