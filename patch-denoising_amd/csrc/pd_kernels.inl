@@ -420,22 +420,35 @@ template <int W> constexpr bool tri16_uses(int b) {
   return false;
 }
 
-// acc[t] += S[:, 16 I_t ..][k] S[:, 16 J_t ..][k]^T over reduction indices
-// k < kcount (a multiple of 4 may run past kcount: those staged rows are
-// zero) on mfma_f32_16x16x4f32.  S is staged [k][i] with row length ldw.
-// Each 16-index block the wave needs is read once per step of 4.
-template <int W>
+// acc[t] += Xc[16 I_t .., k] Xc[16 J_t .., k]^T over reduction indices
+// k < kcount on mfma_f32_16x16x4f32, centring while the operands are read:
+// S holds the raw staged values [k][i] (row length ldw), Xc = S - mean with
+// the mean by reduction index (M-side: per frame) or by Gram index (T-side);
+// indices k >= kcount contribute zero.  Each 16-index block the wave needs is
+// read once per step of 4.
+template <int W, bool MSIDE>
 __device__ __forceinline__ void tri16_accum(const float* __restrict__ S, int ldw, int kcount,
-                                            int lane, floatx4 (&acc)[9]) {
-  for (int k = 0; k < kcount; k += 4) {
-    const float* row = S + (k + (lane >> 4)) * ldw + (lane & 15);
-    float v[8];
+                                            int lane, const float* __restrict__ mean,
+                                            floatx4 (&acc)[9]) {
+  float mi[8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) v[b] = tri16_uses<W>(b) ? row[16 * b] : 0.f;
+  for (int b = 0; b < 8; ++b) mi[b] = (!MSIDE && tri16_uses<W>(b)) ? mean[16 * b + (lane & 15)] : 0.f;
+  auto load = [&](int k, float (&v)[8]) {
+    const int kk = k + (lane >> 4);
+    const bool kv = kk < kcount;
+    const float* row = S + kk * ldw + (lane & 15);
+    const float mk = MSIDE ? mean[kk] : 0.f;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      v[b] = (tri16_uses<W>(b) && kv) ? row[16 * b] - (MSIDE ? mk : mi[b]) : 0.f;
+  };
+  for (int k = 0; k < kcount; k += 4) {
+    float cur[8];
+    load(k, cur);
 #pragma unroll
     for (int t = 0; t < 9; ++t)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[Tri16<W>::I[t]], v[Tri16<W>::J[t]], acc[t],
-                                                    0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[Tri16<W>::I[t]], cur[Tri16<W>::J[t]],
+                                                    acc[t], 0, 0, 0);
   }
 }
 
@@ -525,13 +538,14 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
               mc[a2][b2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la[a2], hb[b2], mc[a2][b2], 0, 0, 0);
             }
         }
-      } else if constexpr (M16) {
+      } else if constexpr (M16) {  // Sk: the raw staging, centred on the fly
         const float* S = reinterpret_cast<const float*>(Sk);
+        const float* mf = reinterpret_cast<const float*>(mean);
         switch (wv) {
-          case 0: tri16_accum<0>(S, LDS_W, kcount, lane, m16); break;
-          case 1: tri16_accum<1>(S, LDS_W, kcount, lane, m16); break;
-          case 2: tri16_accum<2>(S, LDS_W, kcount, lane, m16); break;
-          default: tri16_accum<3>(S, LDS_W, kcount, lane, m16); break;
+          case 0: tri16_accum<0, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
+          case 1: tri16_accum<1, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
+          case 2: tri16_accum<2, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
+          default: tri16_accum<3, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
         }
       }
     };
@@ -568,6 +582,11 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
           if (a.MEANW) reinterpret_cast<IN*>(a.MEANW)[(size_t)p * T + c0 + tid] = mv;
         }
         lds_barrier();
+        if constexpr (M16) {  // centred while the MFMA operands are read
+          accum(Sc, tc);
+          lds_barrier();
+          continue;
+        }
         for (int q = tid; q < SC * NB; q += kNT) {
           const int t = q / NB, r = q % NB;
           const IN v = Sraw[t * LDS_W + r];
@@ -639,6 +658,11 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
         store();
         lds_barrier();
         if (r0 + SC < M) load(r0 + SC);
+        if constexpr (M16) {  // centred while the MFMA operands are read
+          accum(Sc, rc);
+          lds_barrier();
+          continue;
+        }
         for (int q = tid; q < SC * NB; q += kNT) {
           const int r = q / NB, t = q % NB;
           E c = E(R(0));
