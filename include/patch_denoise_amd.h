@@ -182,6 +182,12 @@ typedef struct pdamd_problem {
      patch order; the others keep the FP32 decision.  0 = off.             */
   double refine_rel;
   int32_t* refined_dev;  /* optional, per selected patch: 1 where re-checked */
+
+  /* 1 when the reference's prior variance map is float32 (a float32
+     noise-level array: lowrank.py:94-97, :351-354): each patch's prior
+     (Hybrid: mean variance; Optimal, Adaptive: mean standard deviation) is
+     rounded to float32, the precision the reference averages it in.      */
+  int var_f32;
 } pdamd_problem;
 
 enum { PDAMD_RECOMBINE_AUTO = 0, PDAMD_RECOMBINE_SCATTER = 1, PDAMD_RECOMBINE_GATHER = 2 };

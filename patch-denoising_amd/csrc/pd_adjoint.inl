@@ -237,6 +237,7 @@ struct AdjArgs {
   int loss;
   double mpm, beta;
   const double* var_map;
+  int var_f32;  // round the patch prior to float32 (float32 noise-level map)
   const double* LAM;    // per linear patch index, or null (= 1)
   double* GLAM;         // per linear patch index, or null
   void* gin;            // dL/d(input), input dtype, nvox x T
@@ -388,6 +389,7 @@ __global__ __launch_bounds__(kNT) void adj_spectral_kernel(AdjArgs a) {
         double acc = 0.0;
         for (int r = tid; r < g.M; r += kWave) acc += sqrt(a.var_map[(sp + row_offset(g, r)) / g.T]);
         sigma = dpp_sum(acc) / (double)g.M;
+        if (a.var_f32) sigma = (double)(float)sigma;
       } else {
         RT med;
         if (n % 2) {

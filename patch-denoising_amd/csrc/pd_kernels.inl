@@ -1536,6 +1536,7 @@ struct EigArgs {
   double tau0_log;
   const double* qut_tau;         // QUT: tau by linear patch index
   const double* var_map;
+  int var_f32;        // round the patch prior to float32 (float32 noise-level map)
   const double* LAM;  // optimal rule: per-linear-patch scale of the noise level c (null = 1)
   const void* H;
   const void* DE;
@@ -2112,6 +2113,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
         acc += (a.method == PDAMD_OPTIMAL) ? sqrt(v) : v;
       }
       prior = dpp_sum(acc) / (double)a.g.M;
+      if (a.var_f32) prior = (double)(float)prior;
     }
 
     int ncomp = 0, rank = 0;
@@ -3208,6 +3210,7 @@ EigArgs eig_args(const pdamd_problem* P, const Plan& pl, char* ws, const GramArg
   ea.mpm = P->mp_median;
   ea.beta = (double)pl.g.pt / (double)pl.g.M;
   ea.var_map = P->var_map_dev;
+  ea.var_f32 = P->var_f32;
   ea.LAM = P->lam_dev;
   ea.adaptive_method = P->adaptive_method;
   ea.n_gamma = P->n_gamma;
@@ -3577,7 +3580,7 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
                  const float* w4, void* gin, double* glam, hipStream_t st, StageTimer& tm) {
   const int32_t* idx_all = reinterpret_cast<const int32_t*>(ws + pl.off_idx);
   {
-    AdjArgs ha;
+    AdjArgs ha{};
     ha.g = pl.g;
     ha.recomb = P->recombination;
     ha.mask_kind = P->mask_kind;
@@ -3644,6 +3647,7 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
     aa.mpm = P->mp_median;
     aa.beta = (double)pl.g.pt / (double)pl.g.M;
     aa.var_map = P->var_map_dev;
+    aa.var_f32 = P->var_f32;
     aa.LAM = P->lam_dev;
     aa.GLAM = glam;
     aa.gin = gin;

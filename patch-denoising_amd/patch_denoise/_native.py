@@ -73,6 +73,7 @@ class Problem(ctypes.Structure):
         ("recombine_mode", ctypes.c_int),
         ("refine_rel", ctypes.c_double),
         ("refined_dev", ctypes.c_void_p),
+        ("var_f32", ctypes.c_int),
     ]
 
 

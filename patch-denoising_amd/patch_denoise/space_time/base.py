@@ -284,6 +284,15 @@ def _progress(progbar, n):
         progbar.close()
 
 
+def prior_is_float32(noise_std):
+    """True when the reference's prior variance array is float32 (a float32
+    noise-level array squared: lowrank.py:94-97), so that it averages each
+    patch's prior in float32; a float noise level gives float64."""
+    if isinstance(noise_std, (float, np.floating)):
+        return False
+    return np.asarray(noise_std).dtype == np.float32
+
+
 def spatial_variance_map(noise_std, data_shape):
     """Prior variance per voxel as the reference builds it (lowrank.py:94-101).
 

@@ -13,7 +13,7 @@ import numpy as np
 from scipy.linalg import svd
 
 from .. import _native as N
-from .base import BaseSpaceTimeDenoiser, PatchedArray, spatial_variance_map
+from .base import BaseSpaceTimeDenoiser, PatchedArray, prior_is_float32, spatial_variance_map
 from .utils import marchenko_pastur_median
 
 
@@ -67,9 +67,11 @@ class HybridPCADenoiser(BaseSpaceTimeDenoiser):
             np.broadcast_to(var_map[..., None], input_data.shape), p_s, p_o
         )
         self._var_map = var_map
+        self._var_f32 = prior_is_float32(noise_std)
         return super().denoise(input_data, mask, mask_threshold, progbar=progbar)
 
     def _configure(self, prob, input_data, p_s):
+        prob.var_f32 = int(getattr(self, "_var_f32", False))
         return self._var_map
 
     def _patch_processing(self, patch, patch_idx=None, var_apriori=None):
@@ -179,6 +181,7 @@ class OptimalSVDDenoiser(BaseSpaceTimeDenoiser):
             eps=eps_marshenko_pastur,
         )
         self._var_map = None
+        self._var_f32 = noise_std is not None and prior_is_float32(noise_std)
         if noise_std is None:
             self.input_denoising_kwargs["var_apriori"] = None
         else:
@@ -195,6 +198,7 @@ class OptimalSVDDenoiser(BaseSpaceTimeDenoiser):
     def _configure(self, prob, input_data, p_s):
         kw = self.input_denoising_kwargs
         self._set_rule(prob, kw["mp_median"], kw["shrink_func"])
+        prob.var_f32 = int(getattr(self, "_var_f32", False))
         return self._var_map
 
     def _patch_processing(self, patch, patch_idx=None, shrink_func=None, mp_median=None,
@@ -264,6 +268,7 @@ class AdaptiveDenoiser(BaseSpaceTimeDenoiser):
         full = np.broadcast_to(var_apriori[..., None], input_data.shape)
         self.input_denoising_kwargs["var_apriori"] = PatchedArray(full, p_s, p_o)
         self._var_map = np.ascontiguousarray(full[..., 0], dtype=np.float64)
+        self._var_f32 = np.asarray(var_apriori).dtype == np.float32
         self._mask_for_qut = (mask, mask_threshold)
         return super().denoise(input_data, mask, mask_threshold, progbar=progbar)
 
@@ -314,6 +319,7 @@ class AdaptiveDenoiser(BaseSpaceTimeDenoiser):
             mask_arr = None if mask is None else np.asarray(mask)
             qut = self._qut_taus(prob, input_data, mask_arr, npatch)
         self._set_rule(prob, kw["method"], kw["gamma0"], kw["tau0"], qut)
+        prob.var_f32 = int(getattr(self, "_var_f32", False))
         return self._var_map
 
     def _patch_processing(self, patch, patch_idx=None, gamma0=None, tau0=None, var_apriori=None,

@@ -27,6 +27,8 @@ Fixtures written (all small, ``np.savez_compressed``):
 * ``adaptive.npz``     adaptive trace-norm shrinkage (SURE / GSURE / QUT):
                        per-patch (gamma, tau) of the reference's parameter
                        search and end-to-end outputs on two small volumes
+* ``map32_prior.npz``  Hybrid-PCA and optimal shrinkage with a float32
+                       noise-level map (float32 prior means)
 """
 
 import os
@@ -284,12 +286,38 @@ def adaptive():
     _save("adaptive.npz", **arrays)
 
 
+def map32():
+    """Hybrid-PCA and optimal shrinkage with a float32 noise-level map (the
+    reference then averages the prior in float32: lowrank.py:94-106, :379-384)
+    on two of the small volumes."""
+    vols = workloads.golden_volumes()
+    res = {}
+    for name in ("tside_mask", "2d_p6"):
+        spec = vols[name]
+        data, mask = spec["data"], spec.get("mask")
+        sm32 = np.asarray(spec["sigma_map"], np.float32)
+        common = dict(patch_shape=spec["patch_shape"], patch_overlap=spec["patch_overlap"],
+                      mask=mask, mask_threshold=spec.get("mask_threshold", 50), progbar=False)
+        res[name + "__input"] = data
+        res[name + "__sigma_map32"] = sm32
+        if mask is not None:
+            res[name + "__mask"] = mask
+        res[name + "__patch_shape"] = np.asarray(spec["patch_shape"])
+        res[name + "__patch_overlap"] = np.asarray(spec["patch_overlap"])
+        out, w = _run(R.hybrid_pca, data, noise_std=sm32, recombination="weighted", **common)
+        res[name + "__hybrid__out"], res[name + "__hybrid__w"] = out, w
+        out, w = _run(R.optimal_thresholding, data, loss="fro", noise_std=sm32,
+                      recombination="weighted", **common)
+        res[name + "__optimal__out"], res[name + "__optimal__w"] = out, w
+    _save("map32_prior.npz", **res)
+
+
 if __name__ == "__main__":
     import warnings
 
     warnings.simplefilter("ignore")
     which = sys.argv[1:] or ["e2e", "e2e_small_m", "patches", "mpm", "tau", "cfg1", "noise",
-                             "adaptive"]
+                             "adaptive", "map32"]
     if "mpm" in which:
         mp_median_grid()
     if "tau" in which:
@@ -306,3 +334,5 @@ if __name__ == "__main__":
         noise_map()
     if "adaptive" in which:
         adaptive()
+    if "map32" in which:
+        map32()

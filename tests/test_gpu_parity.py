@@ -179,3 +179,28 @@ def test_split_bf16_tside_gram_against_oracle():
     print(f"[gpu-bf16] T-side Gram 100: rel-L2 vs oracle {e:.3g}, vs fp32 path {e32:.3g}", flush=True)
     assert np.array_equal(np.isnan(out), np.isnan(want))
     assert e <= 1e-3 and e32 > 0.0  # the split path really ran (not bit-identical to FP32)
+
+
+def test_float32_noise_map_against_reference_fixture():
+    """Hybrid-PCA and optimal shrinkage with a float32 noise-level map: the
+    reference averages each patch's prior in float32 (the map's dtype); the
+    device rounds its float64 patch mean to float32 the same way."""
+    pd = _pd()
+    z = np.load(os.path.join(GOLDEN, "map32_prior.npz"))
+    for name in ("tside_mask", "2d_p6"):
+        data, sm32 = z[name + "__input"], z[name + "__sigma_map32"]
+        assert sm32.dtype == np.float32
+        mask = z[name + "__mask"] if name + "__mask" in z.files else None
+        ps, po = z[name + "__patch_shape"], z[name + "__patch_overlap"]
+        ps = int(ps) if ps.ndim == 0 else tuple(int(v) for v in ps)
+        po = int(po) if po.ndim == 0 else tuple(int(v) for v in po)
+        common = dict(patch_shape=ps, patch_overlap=po, mask=mask, recombination="weighted")
+        for key, call in (("hybrid", lambda: pd.hybrid_pca(data, noise_std=sm32, **common)),
+                          ("optimal", lambda: pd.optimal_thresholding(
+                              data, loss="fro", noise_std=sm32, **common))):
+            out, w, _, _ = call()
+            ref, ref_w = z[f"{name}__{key}__out"], z[f"{name}__{key}__w"]
+            assert np.array_equal(np.isnan(out), np.isnan(ref))
+            e, ew = rel_l2(out, ref), rel_l2(w, ref_w)
+            print(f"[gpu-map32] {name} {key}: out {e:.2e} weights {ew:.2e}", flush=True)
+            assert e <= 1e-5 and ew <= 1e-6
