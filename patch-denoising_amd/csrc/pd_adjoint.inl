@@ -558,10 +558,12 @@ __global__ __launch_bounds__(kNT) void adj_synth_kernel(const IN* __restrict__ v
 
 // Back-transformation of every tridiagonal eigenvector at once: Y = Q Z with
 // Q = H_0 ... H_{n-2} from the tridiagonalisation (packed Householder rows +
-// tau), Z the n x n eigenvectors of the tridiagonal.  Thread t owns column
-// t >> 1 and rows 2q + (t & 1) (q < 64) of Z in registers; the two lanes of a
-// column combine their halves of each dot product with one DPP swap, so no
-// barrier is needed per reflector.  Reflectors are staged in LDS, kBtKR at a time.
+// tau), Z the n x n eigenvectors of the tridiagonal.  Thread t owns columns
+// 4 (t >> 3) .. + 3 and rows 16 (t & 7) .. + 15 of Z in registers (n <= 128),
+// so each reflector entry read from LDS serves four columns; the eight lanes
+// of a column group combine their partial dot products with three DPP steps,
+// so no barrier is needed per reflector.  Reflectors are staged in LDS, kBtKR
+// at a time.
 constexpr int kBtKR = 32;
 
 struct BtArgs {
@@ -572,23 +574,33 @@ struct BtArgs {
   void* Y;          // np x n x n
 };
 
+// Sum over the 8 lanes 8a .. 8a + 7; every lane of the group gets the same bits.
+template <class R> __device__ __forceinline__ R sum8_(R v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  return v;
+}
+
 template <class R>
 __global__ __launch_bounds__(kNT) void backtransform_kernel(BtArgs a) {
   const int n = a.n, tid = threadIdx.x;
-  const int j = tid >> 1, h = tid & 1;
-  __shared__ __align__(16) R vs[2][kBtKR][64];
+  const int g = tid & 7, c0 = 4 * (tid >> 3);  // rows 16 g .., columns c0 ..
+  __shared__ __align__(16) R vs[kBtKR][128];
   __shared__ R ts[kBtKR];
   for (int p = blockIdx.x; p < a.np; p += gridDim.x) {
     const R* Hp = reinterpret_cast<const R*>(a.H) + (size_t)p * n * n;
     const R* Tp = reinterpret_cast<const R*>(a.TAU) + (size_t)p * n;
     const R* Zp = reinterpret_cast<const R*>(a.Z) + (size_t)p * n * n;
     R* Yp = reinterpret_cast<R*>(a.Y) + (size_t)p * n * n;
-    R y[64];
+    R y[4][16];
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-      const int r = 2 * q + h;
-      y[q] = (r < n && j < n) ? Zp[(size_t)r * n + j] : R(0);
-    }
+    for (int q = 0; q < 16; ++q)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int r = 16 * g + q, col = c0 + c;
+        y[c][q] = (r < n && col < n) ? Zp[(size_t)r * n + col] : R(0);
+      }
     for (int kh = n - 2; kh >= 0; kh -= kBtKR) {  // reflectors kh, kh-1, ..., kh-kBtKR+1
       __syncthreads();
       for (int e = tid; e < kBtKR * 128; e += kNT) {
@@ -598,27 +610,36 @@ __global__ __launch_bounds__(kNT) void backtransform_kernel(BtArgs a) {
           const R* row = Hp + hh_row(k, n) - (k + 1);
           v = r > k + 1 ? row[r] : (r == k + 1 ? R(1) : R(0));
         }
-        vs[r & 1][kk][r >> 1] = v;
+        vs[kk][r] = v;
       }
       if (tid < kBtKR) ts[tid] = (kh - tid >= 0) ? Tp[kh - tid] : R(0);
       __syncthreads();
       const int kcount = min(kBtKR, kh + 1);
       for (int kk = 0; kk < kcount; ++kk) {
-        const R* v = vs[h][kk];
-        R sdot = R(0);
+        R v[16];
 #pragma unroll
-        for (int q = 0; q < 64; ++q) sdot = fma(v[q], y[q], sdot);
-        sdot += dpp_f<0xB1>(sdot);
-        const R ts_ = ts[kk] * sdot;
+        for (int q = 0; q < 16; ++q) v[q] = vs[kk][16 * g + q];
+        R sd[4];
 #pragma unroll
-        for (int q = 0; q < 64; ++q) y[q] = fma(-ts_, v[q], y[q]);
+        for (int c = 0; c < 4; ++c) {
+          R t = R(0);
+#pragma unroll
+          for (int q = 0; q < 16; ++q) t = fma(v[q], y[c][q], t);
+          sd[c] = ts[kk] * sum8_(t);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int q = 0; q < 16; ++q) y[c][q] = fma(-sd[c], v[q], y[c][q]);
       }
     }
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-      const int r = 2 * q + h;
-      if (r < n && j < n) Yp[(size_t)r * n + j] = y[q];
-    }
+    for (int q = 0; q < 16; ++q)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int r = 16 * g + q, col = c0 + c;
+        if (r < n && col < n) Yp[(size_t)r * n + col] = y[c][q];
+      }
   }
 }
 

@@ -1948,16 +1948,15 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
     if (c < n) {
       R* z = Z + c;
       R* dp = scr + c;                   // D+ of L+ D+ L+^T = T - lam I (top-down)
-      R* lp = scr + (size_t)n * ks + c;  // L+
       R* um = scr + (size_t)2 * n * ks + c;  // U- of U- D- U-^T (bottom-up)
       const R lm = lam[c];
       auto guard = [&](R x) { return fabs(x) < eps_piv ? copysign(eps_piv, x) : x; };
-      // twisted factorization: z = (T - lam I)^-1 e_r up to scale, r = argmin |gamma_r|
+      // twisted factorization: z = (T - lam I)^-1 e_r up to scale, r = argmin |gamma_r|;
+      // L+ (= e_i / D+_i) is recomputed from the stored D+ where z needs it
       R dv = guard(d[0] - lm);
       dp[0] = dv;
       for (int i = 0; i + 1 < n; ++i) {
         const R l = qdiv_(e[i], dv);
-        lp[(size_t)i * ks] = l;
         dv = guard((d[i + 1] - lm) - l * e[i]);
         dp[(size_t)(i + 1) * ks] = dv;
       }
@@ -1977,7 +1976,7 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
       z[(size_t)r * ks] = R(1);
       R zi = R(1), ss = R(1);
       for (int i = r - 1; i >= 0; --i) {
-        zi = -lp[(size_t)i * ks] * zi;
+        zi = -qdiv_(e[i], dp[(size_t)i * ks]) * zi;
         z[(size_t)i * ks] = zi;
         ss += zi * zi;
       }
@@ -1995,7 +1994,7 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
           z[(size_t)i * ks] = R(0.5) + (R)(hs & 0xFFFF) * R(1.0 / 65536.0);
         }
         for (int it = 0; it < 2; ++it) {
-          tri_solve<R, true>(d, e, n, lm, dp, lp, um, z, ks, eps_piv);
+          tri_solve<R, true>(d, e, n, lm, dp, scr + (size_t)n * ks + c, um, z, ks, eps_piv);
           R mx = R(0);
           for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(z[(size_t)i * ks]));
           const R inv = mx > R(0) ? R(1) / mx : R(1);
