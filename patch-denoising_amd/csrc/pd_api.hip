@@ -134,11 +134,11 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
   // rule runs on FP32 eigenvalues
   pl->refine = !bwd && P->refine_rel > 0 && P->dtype == PDAMD_F32 && pl->etype == 0 &&
                (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID);
-  // chunk: keep the per-chunk buffers under ~4 GiB (6 GiB for the backward)
+  // chunk: keep the per-chunk buffers under ~4 GiB (24 GiB for the backward)
   size_t per = (size_t)n * n * pl->ebytes + (size_t)n * pl->kmax * (pl->ebytes + pl->rbytes) +
                3 * n * pl->ebytes + pl->kmax * 8 + 16;
   if (bwd) per += (3 * (size_t)M * n + (size_t)n * n + 2 * (size_t)n) * pl->ebytes + 8;
-  int64_t chunk = (int64_t)((bwd ? (6ull << 30) : (4ull << 30)) / per);
+  int64_t chunk = (int64_t)((bwd ? (24ull << 30) : (12ull << 30)) / per);
   chunk = std::max<int64_t>(256, std::min<int64_t>(chunk, 65536));
   pl->chunk = std::min<int64_t>(chunk, std::max<int64_t>(pl->npatch, 1));
   if (chunk_cap > 0) pl->chunk = std::min<int64_t>(pl->chunk, chunk_cap);

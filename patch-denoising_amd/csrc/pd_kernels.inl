@@ -3379,6 +3379,15 @@ int run_refine(pdamd_problem* P, const Plan& pl, char* ws, const uint8_t* flags,
 // Householder and eigen stages write the per-call means / eigenvectors /
 // counts, the factor kernel the per-patch rows; then one pass builds the
 // patch map, the float32 weight sums and every output element.
+// Patches per chunk for `count` selected patches: the plan's maximum, evened
+// out so that the last chunk is not a small remainder (every launch then
+// fills the GPU alike).
+inline int64_t even_chunk(const Plan& pl, int64_t count) {
+  if (count <= 0) return pl.chunk;
+  const int64_t nch = (count + pl.chunk - 1) / pl.chunk;
+  return (count + nch - 1) / nch;
+}
+
 int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStream_t st,
                       StageTimer& tm);
 
@@ -3395,8 +3404,9 @@ int run_chunks(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, hipStr
   double* wacc = reinterpret_cast<double*>(ws + pl.off_w3);
   const bool w3d = pl.g.gt == 1;
   uint8_t* rflags = reinterpret_cast<uint8_t*>(ws + pl.off_RFLAG);
-  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
-    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+  const int64_t chunk = even_chunk(pl, count);
+  for (int64_t c0 = 0; c0 < count; c0 += chunk) {
+    const int np = (int)std::min<int64_t>(chunk, count - c0);
     GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
     tm.begin(PDAMD_STAGE_GRAM);
     int rc = pl.tside ? launch_gram_tile<IN, E, true>(pl, P->input_dev, ga, st)
@@ -3498,8 +3508,9 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
   fa.mask_kind = P->mask_kind;
   fa.out = reinterpret_cast<float*>(P->output_dev);
   fa.wout = P->weights_dev;
-  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
-    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+  const int64_t chunk = even_chunk(pl, count);
+  for (int64_t c0 = 0; c0 < count; c0 += chunk) {
+    const int np = (int)std::min<int64_t>(chunk, count - c0);
     GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
     ga.MEANW = ws + pl.off_GMEAN + (size_t)c0 * T * sizeof(float);
     tm.begin(PDAMD_STAGE_GRAM);
@@ -3593,8 +3604,9 @@ int run_backward(pdamd_problem* P, const Plan& pl, char* ws, int64_t count, cons
     tm.end();
     HIP_TRY(hipGetLastError());
   }
-  for (int64_t c0 = 0; c0 < count; c0 += pl.chunk) {
-    const int np = (int)std::min<int64_t>(pl.chunk, count - c0);
+  const int64_t chunk = even_chunk(pl, count);
+  for (int64_t c0 = 0; c0 < count; c0 += chunk) {
+    const int np = (int)std::min<int64_t>(chunk, count - c0);
     // recompute the Gram eigendecomposition (all pairs)
     GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
     tm.begin(PDAMD_STAGE_GRAM);
