@@ -233,15 +233,20 @@ def bench_differentiable(steps):
             fwd()
     torch.cuda.synchronize()
     t_f = (time.perf_counter() - t0) / steps
-    ms = (ctypes.c_float * len(N.STAGES))()
-    cnt = (ctypes.c_int64 * len(N.STAGES))()
-    spec.timers = (ms, cnt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / steps
+    # per-stage breakdown from separate steps (the HIP-event timers add a
+    # synchronisation per call, so they stay out of the timed loop)
+    ms = (ctypes.c_float * len(N.STAGES))()
+    cnt = (ctypes.c_int64 * len(N.STAGES))()
+    spec.timers = (ms, cnt)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
     spec.timers = None
     ok = bool(torch.isfinite(x.grad).all() and torch.isfinite(lam.grad).all())
     vf = float(np.prod(noisy.shape))
