@@ -292,6 +292,16 @@ def bench_batched(steps, autotune=True):
     t_eager = timed(eager.run)
     t_graph = timed(graph.run)
     del graph
+    # error of every volume against the same call on its float64 cast (the
+    # FP64 path end to end)
+    from patch_denoise import denoise as F
+
+    outs32 = eager.results()
+    errs = []
+    for (name, args, kw), o32 in zip(calls, outs32):
+        o64 = getattr(F, name)(args[0].astype(np.float64), *args[1:], **kw)
+        errs.append(_rel_l2(np.asarray(o32[0]), np.asarray(o64[0])))
+        del o64
     tuned_res = {}
     if autotune:
         t0 = time.perf_counter()
@@ -305,6 +315,7 @@ def bench_batched(steps, autotune=True):
         dict(method=c[0], shape=list(c[1][0].shape), patch=c[1][1]) for c in calls],
         skipped=[dict(index=i, reason=r) for i, r in skipped], voxel_frames=vf,
         patches=eager.batch.selected, steps=steps, datagen_s=gen_s,
+        rel_l2_vs_fp64=errs,
         sequential_ms=t_seq * 1e3, batch_ms=t_eager * 1e3, batch_graph_ms=t_graph * 1e3,
         sequential_voxel_frames_s=vf / t_seq, batch_graph_voxel_frames_s=vf / t_graph,
         speedup_graph_vs_sequential=t_seq / t_graph, **tuned_res)
