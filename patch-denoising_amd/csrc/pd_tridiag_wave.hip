@@ -265,18 +265,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   // every load of the Gram has returned before reflector rows overwrite it
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
-  // phases of 16 steps, two kept block rows / columns fewer each (two block
-  // shifts per phase), down to 2; phase widths are compile-time counts
+  // phases of 8 steps, one kept block row / column fewer each (one block
+  // shift per phase), down to 1; phase widths are compile-time counts
   int k = 0, base = 0;
   const int last = n - 1;
-  if constexpr (NB >= 16) tri_phase<R, NB, 16>(L, k, min(NB * 8 - 112, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 14) tri_phase<R, NB, 14>(L, k, min(NB * 8 - 96, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 12) tri_phase<R, NB, 12>(L, k, min(NB * 8 - 80, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 10) tri_phase<R, NB, 10>(L, k, min(NB * 8 - 64, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 8) tri_phase<R, NB, 8>(L, k, min(NB * 8 - 48, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 6) tri_phase<R, NB, 6>(L, k, min(NB * 8 - 32, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  if constexpr (NB >= 4) tri_phase<R, NB, 4>(L, k, min(NB * 8 - 16, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
-  tri_phase<R, NB, 2>(L, k, last, n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 16) tri_phase<R, NB, 16>(L, k, min(NB * 8 - 120, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 15) tri_phase<R, NB, 15>(L, k, min(NB * 8 - 112, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 14) tri_phase<R, NB, 14>(L, k, min(NB * 8 - 104, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 13) tri_phase<R, NB, 13>(L, k, min(NB * 8 - 96, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 12) tri_phase<R, NB, 12>(L, k, min(NB * 8 - 88, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 11) tri_phase<R, NB, 11>(L, k, min(NB * 8 - 80, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 10) tri_phase<R, NB, 10>(L, k, min(NB * 8 - 72, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 9) tri_phase<R, NB, 9>(L, k, min(NB * 8 - 64, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 8) tri_phase<R, NB, 8>(L, k, min(NB * 8 - 56, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 7) tri_phase<R, NB, 7>(L, k, min(NB * 8 - 48, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 6) tri_phase<R, NB, 6>(L, k, min(NB * 8 - 40, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 5) tri_phase<R, NB, 5>(L, k, min(NB * 8 - 32, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 4) tri_phase<R, NB, 4>(L, k, min(NB * 8 - 24, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 3) tri_phase<R, NB, 3>(L, k, min(NB * 8 - 16, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  if constexpr (NB >= 2) tri_phase<R, NB, 2>(L, k, min(NB * 8 - 8, last), n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
+  tri_phase<R, NB, 1>(L, k, last, n, base, Vt, Pa, Pb, dd, ee, tt, Hp);
   {
     const int il = (n - 1) - base;  // 0 .. 7
     const R dl = readlane_r(L[tri_ix(0, 0)], 9 * il);
