@@ -3116,6 +3116,22 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       return PDAMD_OK;
     }
   }
+  // real FP32 Grams of 129 .. 208: four wavefronts per patch, tiles in registers
+  if constexpr (std::is_same<E, float>::value) {
+    if (pl.tile == 0 && tridiag_wg_supported(pl.n, false)) {
+      TridiagWaveArgs ta;
+      ta.H = ga.H;
+      ta.DE = ga.DE;
+      ta.TAU = ga.TAU;
+      ta.np = ga.np;
+      ta.n = pl.n;
+      ta.np_dev = ga.np_dev;
+      if (launch_tridiag_wg(ta, st))
+        return fail(PDAMD_EHIP, "tridiagonalisation launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+      return PDAMD_OK;
+    }
+  }
   switch (pl.tile) {
     case 0: return launch_tridiag_big<E>(pl, ga, st);
     case 2: return launch_tridiag<E, 8>(pl, ga, st);

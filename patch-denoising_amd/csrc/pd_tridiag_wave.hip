@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "pd_common.h"
 #include "pd_tridiag.h"
@@ -302,6 +303,210 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup variant for Grams of 129 .. 16 NBT: four wavefronts per patch in a
+// 2 x 2 grid (wa, wb).  The matrix is cut into 16 x 16 tiles; lane (a, b) of
+// wavefront (wa, wb) holds element (8 wa + a, 8 wb + b) of every tile (s, t)
+// with t <= s, in registers (NBT (NBT + 1) / 2 per lane: 91 at n <= 208).
+// Diagonal tiles are kept whole, so every wavefront has the same compile-time
+// register layout and no per-tile branches.
+// Step k (column k in tile column P = k / 16, held by the wavefronts with
+// wb = (k / 8) % 2 in lanes b = k % 8):
+//   A  the owners write column k to LDS (zero at rows <= k + 1) with their
+//      partial |x|^2, alpha = A[k+1][k] and d_k; barrier;
+//   B  every lane forms beta, tau and v from the same LDS values (identical
+//      bits); p = A v over its tiles: off-diagonal tiles add to the row side
+//      (summed over b) and the column side (summed over a), diagonal tiles to
+//      the row side; each index i then has two row parts (wb = 0, 1) and two
+//      column parts (wa = 0, 1), written to LDS, and each wavefront its part
+//      of v . p; barrier;
+//   C  p_i = the four parts added in one fixed order, so the row and column
+//      copies of w agree bit for bit; w = tau p - (tau^2 / 2)(v . p) v, zero at
+//      eliminated indices; A -= v w^T + w v^T on the kept tiles.
+// Phases of 16 steps start at tile column P (compile-time ranges).
+// Outputs as the LDS kernel for large Grams: v_k in row k of the patch's H
+// block (columns k+2 .. n-1, v_k[k+1] = 1 implicit), d | e | e^2, tau.
+// LDS vectors hold index i at (i % 16) LD + i / 16 (LD = 4 mod 8: the 8 lanes
+// of a row or column group read 8 different banks).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int wg_ld(int nb) { return (nb & 7) == 4 ? nb : wg_ld(nb + 1); }
+
+struct WgLds {
+  float* X;    // [16][LD]: column k
+  float* RS;   // [2][16][LD]: row parts of p, by wb
+  float* CS;   // [2][16][LD]: column parts of p, by wa
+  float* NRM;  // [2]: |x|^2 parts, by wa
+  float* DOT;  // [4]: v . p parts, by wavefront
+  float* AL;   // [1]: alpha
+  float* dd;   // [16 NBT]
+  float* ee;   // [16 NBT]
+  float* tt;   // [16 NBT]
+};
+
+template <int NBT, int P>
+__device__ __forceinline__ void wg_phase(float (&L)[NBT * (NBT + 1) / 2], int& k, int last, int n,
+                                         int wa, int wb, int wid, const WgLds& s, float* Hp) {
+  using R = float;
+  constexpr int LD = wg_ld(NBT), SL = 16 * LD;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int a = lane >> 3, b = lane & 7;
+  const int ra = 8 * wa + a, cb = 8 * wb + b;  // coordinates inside a tile
+  const int kend = min(16 * P + 16, last);
+  for (; k < kend; ++k) {
+    const int u = k & 7;
+    // ---- A: column k below the diagonal (wavefronts wb == (k / 8) % 2)
+    if (wb == ((k >> 3) & 1)) {
+      R sq = R(0);
+#pragma unroll
+      for (int t = P; t < NBT; ++t) {
+        const int i = 16 * t + ra;
+        const R x = L[tri_ix(t, P)];
+        const R xv = i > k + 1 ? x : R(0);
+        sq = fma(xv, xv, sq);
+        if (b == u) {
+          s.X[ra * LD + t] = xv;
+          if (i == k + 1) *s.AL = x;
+          if (i == k) s.dd[k] = x;
+        }
+      }
+      sq = sum8_a(sq);
+      if (lane == u) s.NRM[wa] = sq;
+    }
+    lds_barrier();
+    // ---- B: reflector, v, p = A v
+    const R xn2 = s.NRM[0] + s.NRM[1];
+    const R alpha = *s.AL;
+    R beta = alpha, tk = R(0), scal = R(0);
+    if (xn2 > R(0)) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tk = (beta - alpha) / beta;
+      scal = R(1) / (alpha - beta);
+    }
+    const R one = xn2 > R(0) ? R(1) : R(0);
+    if (threadIdx.x == 0) {
+      s.ee[k] = beta;
+      s.tt[k] = tk;
+    }
+    R vr[NBT], vc[NBT];
+#pragma unroll
+    for (int t = P; t < NBT; ++t) {
+      vr[t] = s.X[ra * LD + t] * scal;
+      vc[t] = s.X[cb * LD + t] * scal;
+      if (t <= P + 1) {
+        vr[t] = 16 * t + ra == k + 1 ? one : vr[t];
+        vc[t] = 16 * t + cb == k + 1 ? one : vc[t];
+      }
+    }
+    if (wid == (k & 3)) {  // v_k -> row k of H
+      R* row = Hp + (size_t)k * n;
+      for (int i = k + 2 + lane; i < n; i += kWave) row[i] = s.X[(i & 15) * LD + (i >> 4)] * scal;
+    }
+    R pc[NBT];
+    R drow = R(0), dcol = R(0);
+#pragma unroll
+    for (int t = P; t < NBT; ++t) {
+      R pr = L[tri_ix(t, t)] * vc[t];  // diagonal tile: row side
+#pragma unroll
+      for (int q = P; q < t; ++q) {
+        const R x = L[tri_ix(t, q)];
+        pr = fma(x, vc[q], pr);
+        pc[q] = q + 1 == t ? x * vr[t] : fma(x, vr[t], pc[q]);  // first term of column q at t = q + 1
+      }
+      pr = sum8_b(pr);
+      if (b == 0) s.RS[wb * SL + ra * LD + t] = pr;
+      drow = fma(pr, vr[t], drow);
+    }
+#pragma unroll
+    for (int q = P; q + 1 < NBT; ++q) {
+      const R c = sum8_a(pc[q]);
+      if (a == 0) s.CS[wa * SL + cb * LD + q] = c;
+      dcol = fma(c, vc[q], dcol);
+    }
+    if (a == 0) s.CS[wa * SL + cb * LD + NBT - 1] = R(0);
+    R dp = (b == 0 ? drow : R(0)) + (a == 0 ? dcol : R(0));
+    dp = wave_total(dp);
+    if (lane == 0) s.DOT[wid] = dp;
+    lds_barrier();
+    // ---- C: w, rank-2 update
+    const R dsum = (s.DOT[0] + s.DOT[1]) + (s.DOT[2] + s.DOT[3]);
+    const R kc = R(-0.5) * tk * (tk * dsum);
+    auto p_at = [&](int o) { return ((s.RS[o] + s.RS[SL + o]) + s.CS[o]) + s.CS[SL + o]; };
+    R wr[NBT];
+#pragma unroll
+    for (int t = P; t < NBT; ++t) {
+      wr[t] = fma(kc, vr[t], tk * p_at(ra * LD + t));
+      if (t == P) wr[t] = 16 * t + ra > k ? wr[t] : R(0);
+    }
+#pragma unroll
+    for (int q = P; q < NBT; ++q) {
+      R wc = fma(kc, vc[q], tk * p_at(cb * LD + q));
+      if (q == P) wc = 16 * q + cb > k ? wc : R(0);
+#pragma unroll
+      for (int t = q; t < NBT; ++t) {
+        R x = L[tri_ix(t, q)];
+        x = fma(-vr[t], wc, x);
+        x = fma(-wr[t], vc[q], x);
+        L[tri_ix(t, q)] = x;
+      }
+    }
+  }
+}
+
+template <int NBT, int... Ps>
+__device__ __forceinline__ void wg_phases(std::integer_sequence<int, Ps...>, float (&L)[NBT * (NBT + 1) / 2],
+                                          int& k, int last, int n, int wa, int wb, int wid,
+                                          const WgLds& s, float* Hp) {
+  (wg_phase<NBT, Ps>(L, k, last, n, wa, wb, wid, s, Hp), ...);
+}
+
+template <int NBT>
+__global__ __launch_bounds__(4 * kWave) void tridiag_wg_kernel(TridiagWaveArgs args) {
+  using R = float;
+  constexpr int NT = NBT * (NBT + 1) / 2, LD = wg_ld(NBT), SL = 16 * LD;
+  const int n = args.n;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int a = lane >> 3, b = lane & 7;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int wa = wid >> 1, wb = wid & 1;
+  __shared__ R xs[SL], rs[2 * SL], cs[2 * SL], nrm[2], dot[4], al[1];
+  __shared__ R dd[16 * NBT], ee[16 * NBT], tt[16 * NBT];
+  const WgLds s{xs, rs, cs, nrm, dot, al, dd, ee, tt};
+
+  const int p = blockIdx.x;
+  if (args.np_dev && p >= *args.np_dev) return;
+  R* Hp = reinterpret_cast<R*>(args.H) + (size_t)p * n * n;
+  R L[NT];
+#pragma unroll
+  for (int t = 0; t < NBT; ++t)
+#pragma unroll
+    for (int q = 0; q <= t; ++q) {
+      const int i = 16 * t + 8 * wa + a, c = 16 * q + 8 * wb + b;
+      L[tri_ix(t, q)] = (i < n && c < n) ? Hp[(size_t)i * n + c] : R(0);
+    }
+  // every wavefront's Gram loads have returned before reflector rows overwrite H
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+
+  int k = 0;
+  wg_phases<NBT>(std::make_integer_sequence<int, NBT>{}, L, k, n - 1, n, wa, wb, wid, s, Hp);
+  {  // d_{n-1}: diagonal tile (n-1) / 16, element ((n-1) % 16, (n-1) % 16)
+    const int tl = (n - 1) >> 4, cl = (n - 1) & 15;
+#pragma unroll
+    for (int t = 0; t < NBT; ++t)
+      if (t == tl && wa == (cl >> 3) && wb == (cl >> 3) && lane == 9 * (cl & 7)) dd[n - 1] = L[tri_ix(t, t)];
+  }
+  __syncthreads();
+  R* DEp = reinterpret_cast<R*>(args.DE) + (size_t)p * 3 * n;
+  R* TAUp = reinterpret_cast<R*>(args.TAU) + (size_t)p * n;
+  for (int q = threadIdx.x; q < n; q += 4 * kWave) {
+    const R ev = q < n - 1 ? ee[q] : R(0);
+    DEp[q] = dd[q];
+    DEp[n + q] = ev;
+    DEp[2 * n + q] = ev * ev;
+    TAUp[q] = q < n - 1 ? tt[q] : R(0);
+  }
+}
+
 template <class R, int NB>
 int launch_nb(const TridiagWaveArgs& a, hipStream_t st) {
   const size_t lds = 6 * 8 * NB * sizeof(R);
@@ -314,6 +519,18 @@ int launch_nb(const TridiagWaveArgs& a, hipStream_t st) {
 }  // namespace
 
 namespace pd {
+
+int tridiag_wg_supported(int n, bool f64) { return !f64 && n > 128 && n <= 16 * kWgNT; }
+
+int launch_tridiag_wg(const TridiagWaveArgs& a, hipStream_t st) {
+  if (!tridiag_wg_supported(a.n, false)) return -2;
+  if (a.np < 1) return 0;
+  // tiles per side: the smallest instantiation that holds the Gram
+  auto kern = a.n <= 144 ? tridiag_wg_kernel<9> : a.n <= 176 ? tridiag_wg_kernel<11>
+                                                             : tridiag_wg_kernel<kWgNT>;
+  kern<<<a.np, 4 * kWave, 0, st>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int tridiag_wave_nb(int n, bool f64) {
   return n <= 32 ? 4 : (n <= 64 ? 8 : (f64 ? 0 : (n <= 96 ? 12 : (n <= 128 ? 16 : 0))));

@@ -47,6 +47,16 @@ def _cases():
     c = []
     # T-side, n = 200 (7^3 = 343 voxels > 200 frames), MP-PCA float32
     c.append(("mppca_f32_p7_T200", "mp_pca", _lowrank((9, 9, 8), 200, 6, 1), (7, 6), {}))
+    # real float32 Grams of 129 .. 208 (four-wavefront register Householder):
+    # the smallest size, sizes off the 16- and 8-row tile grid, the largest,
+    # and an M-side Hybrid-PCA Gram
+    c.append(("mppca_f32_p7_T131", "mp_pca", _lowrank((8, 8, 8), 131, 4, 12), (7, 6), {}))
+    c.append(("mppca_f32_p7_T150", "mp_pca", _lowrank((9, 9, 8), 150, 5, 10), (7, 6), {}))
+    c.append(("mppca_f32_p7_T180", "mp_pca", _lowrank((8, 9, 8), 180, 5, 14), (7, 6), {}))
+    c.append(("mppca_f32_p7_T208", "mp_pca", _lowrank((8, 8, 9), 208, 5, 13), (7, 6), {}))
+    d = _lowrank((8, 8, 6), 300, 4, 11)
+    c.append(("hybrid_f32_p6x6x4_T300", "hybrid_pca", d, ((6, 6, 4), (5, 5, 3)),
+              {"noise_std": _sigma_map(d.shape[:-1])}))
     # M-side, n = 216 (6^3 voxels < 260 frames), MP-PCA float64
     c.append(("mppca_f64_p6_T260", "mp_pca",
               _lowrank((8, 8, 7), 260, 5, 2, dtype=np.float64), (6, 5), {}))
