@@ -3116,9 +3116,10 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       return PDAMD_OK;
     }
   }
-  // real FP32 Grams of 129 .. 208: four wavefronts per patch, tiles in registers
-  if constexpr (std::is_same<E, float>::value) {
-    if (pl.tile == 0 && tridiag_wg_supported(pl.n, false)) {
+  // real Grams up to 208 (FP32 above 128, FP64 above 64): four wavefronts per
+  // patch, tiles in registers
+  if constexpr (std::is_same<E, float>::value || F64) {
+    if (tridiag_wg_supported(pl.n, F64)) {
       TridiagWaveArgs ta;
       ta.H = ga.H;
       ta.DE = ga.DE;
@@ -3126,7 +3127,7 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       ta.np = ga.np;
       ta.n = pl.n;
       ta.np_dev = ga.np_dev;
-      if (launch_tridiag_wg(ta, st))
+      if (launch_tridiag_wg(ta, F64, st))
         return fail(PDAMD_EHIP, "tridiagonalisation launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
       return PDAMD_OK;

@@ -331,22 +331,24 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) vo
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int wg_ld(int nb) { return (nb & 7) == 4 ? nb : wg_ld(nb + 1); }
 
+template <class R>
 struct WgLds {
-  float* X;    // [16][LD]: column k
-  float* RS;   // [2][16][LD]: row parts of p, by wb
-  float* CS;   // [2][16][LD]: column parts of p, by wa
-  float* NRM;  // [2]: |x|^2 parts, by wa
-  float* DOT;  // [4]: v . p parts, by wavefront
-  float* AL;   // [1]: alpha
-  float* dd;   // [16 NBT]
-  float* ee;   // [16 NBT]
-  float* tt;   // [16 NBT]
+  R* X;    // [16][LD]: column k
+  R* RS;   // [2][16][LD]: row parts of p, by wb
+  R* CS;   // [2][16][LD]: column parts of p, by wa
+  R* NRM;  // [2]: |x|^2 parts, by wa
+  R* DOT;  // [4]: v . p parts, by wavefront
+  R* AL;   // [1]: alpha
+  R* dd;   // [16 NBT]
+  R* ee;   // [16 NBT]
+  R* tt;   // [16 NBT]
 };
 
-template <int NBT, int P>
-__device__ __forceinline__ void wg_phase(float (&L)[NBT * (NBT + 1) / 2], int& k, int last, int n,
-                                         int wa, int wb, int wid, const WgLds& s, float* Hp) {
-  using R = float;
+// PK: reflector rows packed (row k at hh_row_w(k, n), the layout of the eigen
+// stage for n <= 128) instead of row k of the n x n block.
+template <class R, int NBT, bool PK, int P>
+__device__ __forceinline__ void wg_phase(R (&L)[NBT * (NBT + 1) / 2], int& k, int last, int n,
+                                         int wa, int wb, int wid, const WgLds<R>& s, R* Hp) {
   constexpr int LD = wg_ld(NBT), SL = 16 * LD;
   const int lane = threadIdx.x & (kWave - 1);
   const int a = lane >> 3, b = lane & 7;
@@ -397,8 +399,9 @@ __device__ __forceinline__ void wg_phase(float (&L)[NBT * (NBT + 1) / 2], int& k
         vc[t] = 16 * t + cb == k + 1 ? one : vc[t];
       }
     }
-    if (wid == (k & 3)) {  // v_k -> row k of H
-      R* row = Hp + (size_t)k * n;
+    if (wid == (k & 3)) {  // v_k -> row k of H (columns k+2 .. n-1; packed rows also get v_k[k+1])
+      R* row = PK ? Hp + hh_row_w(k, n) - (k + 1) : Hp + (size_t)k * n;
+      if (PK && lane == 0) row[k + 1] = one;
       for (int i = k + 2 + lane; i < n; i += kWave) row[i] = s.X[(i & 15) * LD + (i >> 4)] * scal;
     }
     R pc[NBT];
@@ -452,16 +455,15 @@ __device__ __forceinline__ void wg_phase(float (&L)[NBT * (NBT + 1) / 2], int& k
   }
 }
 
-template <int NBT, int... Ps>
-__device__ __forceinline__ void wg_phases(std::integer_sequence<int, Ps...>, float (&L)[NBT * (NBT + 1) / 2],
+template <class R, int NBT, bool PK, int... Ps>
+__device__ __forceinline__ void wg_phases(std::integer_sequence<int, Ps...>, R (&L)[NBT * (NBT + 1) / 2],
                                           int& k, int last, int n, int wa, int wb, int wid,
-                                          const WgLds& s, float* Hp) {
-  (wg_phase<NBT, Ps>(L, k, last, n, wa, wb, wid, s, Hp), ...);
+                                          const WgLds<R>& s, R* Hp) {
+  (wg_phase<R, NBT, PK, Ps>(L, k, last, n, wa, wb, wid, s, Hp), ...);
 }
 
-template <int NBT>
+template <class R, int NBT, bool PK>
 __global__ __launch_bounds__(4 * kWave) void tridiag_wg_kernel(TridiagWaveArgs args) {
-  using R = float;
   constexpr int NT = NBT * (NBT + 1) / 2, LD = wg_ld(NBT), SL = 16 * LD;
   const int n = args.n;
   const int lane = threadIdx.x & (kWave - 1);
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(4 * kWave) void tridiag_wg_kernel(TridiagWaveArgs a
   const int wa = wid >> 1, wb = wid & 1;
   __shared__ R xs[SL], rs[2 * SL], cs[2 * SL], nrm[2], dot[4], al[1];
   __shared__ R dd[16 * NBT], ee[16 * NBT], tt[16 * NBT];
-  const WgLds s{xs, rs, cs, nrm, dot, al, dd, ee, tt};
+  const WgLds<R> s{xs, rs, cs, nrm, dot, al, dd, ee, tt};
 
   const int p = blockIdx.x;
   if (args.np_dev && p >= *args.np_dev) return;
@@ -488,7 +490,7 @@ __global__ __launch_bounds__(4 * kWave) void tridiag_wg_kernel(TridiagWaveArgs a
   __syncthreads();
 
   int k = 0;
-  wg_phases<NBT>(std::make_integer_sequence<int, NBT>{}, L, k, n - 1, n, wa, wb, wid, s, Hp);
+  wg_phases<R, NBT, PK>(std::make_integer_sequence<int, NBT>{}, L, k, n - 1, n, wa, wb, wid, s, Hp);
   {  // d_{n-1}: diagonal tile (n-1) / 16, element ((n-1) % 16, (n-1) % 16)
     const int tl = (n - 1) >> 4, cl = (n - 1) & 15;
 #pragma unroll
@@ -520,15 +522,24 @@ int launch_nb(const TridiagWaveArgs& a, hipStream_t st) {
 
 namespace pd {
 
-int tridiag_wg_supported(int n, bool f64) { return !f64 && n > 128 && n <= 16 * kWgNT; }
+int tridiag_wg_supported(int n, bool f64) { return n > (f64 ? 64 : 128) && n <= 16 * kWgNT; }
 
-int launch_tridiag_wg(const TridiagWaveArgs& a, hipStream_t st) {
-  if (!tridiag_wg_supported(a.n, false)) return -2;
+int launch_tridiag_wg(const TridiagWaveArgs& a, bool f64, hipStream_t st) {
+  if (!tridiag_wg_supported(a.n, f64)) return -2;
   if (a.np < 1) return 0;
   // tiles per side: the smallest instantiation that holds the Gram
-  auto kern = a.n <= 144 ? tridiag_wg_kernel<9> : a.n <= 176 ? tridiag_wg_kernel<11>
-                                                             : tridiag_wg_kernel<kWgNT>;
-  kern<<<a.np, 4 * kWave, 0, st>>>(a);
+  if (f64) {
+    auto kern = a.n <= 128   ? tridiag_wg_kernel<double, 8, true>
+                : a.n <= 144 ? tridiag_wg_kernel<double, 9, false>
+                : a.n <= 176 ? tridiag_wg_kernel<double, 11, false>
+                             : tridiag_wg_kernel<double, kWgNT, false>;
+    kern<<<a.np, 4 * kWave, 0, st>>>(a);
+  } else {
+    auto kern = a.n <= 144   ? tridiag_wg_kernel<float, 9, false>
+                : a.n <= 176 ? tridiag_wg_kernel<float, 11, false>
+                             : tridiag_wg_kernel<float, kWgNT, false>;
+    kern<<<a.np, 4 * kWave, 0, st>>>(a);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -57,6 +57,14 @@ def _cases():
     d = _lowrank((8, 8, 6), 300, 4, 11)
     c.append(("hybrid_f32_p6x6x4_T300", "hybrid_pca", d, ((6, 6, 4), (5, 5, 3)),
               {"noise_std": _sigma_map(d.shape[:-1])}))
+    # the same reduction in FP64 (Grams of 65 .. 208): raw SVT and optimal
+    # shrinkage on float32 data (FP64 eigen stage by default), float64 data
+    c.append(("raw_f32_p7_T200", "raw_svt", _lowrank((9, 9, 8), 200, 5, 15), (7, 6),
+              {"threshold": 30.0 * (np.sqrt(343) + np.sqrt(200)) * 1.02}))
+    c.append(("optimal_fro_f32_p7_T150", "optimal_thresholding", _lowrank((8, 8, 9), 150, 5, 16),
+              (7, 6), {"loss": "fro"}))
+    c.append(("mppca_f64_p7_T170", "mp_pca", _lowrank((8, 8, 8), 170, 4, 17, dtype=np.float64),
+              (7, 6), {}))
     # M-side, n = 216 (6^3 voxels < 260 frames), MP-PCA float64
     c.append(("mppca_f64_p6_T260", "mp_pca",
               _lowrank((8, 8, 7), 260, 5, 2, dtype=np.float64), (6, 5), {}))
