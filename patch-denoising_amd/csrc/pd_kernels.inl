@@ -3116,10 +3116,13 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       return PDAMD_OK;
     }
   }
-  // real Grams up to 208 (FP32 above 128, FP64 above 64): four wavefronts per
-  // patch, tiles in registers
-  if constexpr (std::is_same<E, float>::value || F64) {
-    if (tridiag_wg_supported(pl.n, F64)) {
+  // Grams up to 208 (real FP32 above 128, real FP64 and complex FP32 above
+  // 64): four wavefronts per patch, tiles in registers
+  constexpr int WK = std::is_same<E, float>::value ? kWgF32
+                     : F64                          ? kWgF64
+                     : std::is_same<E, cpx<float>>::value ? kWgC32 : -1;
+  if constexpr (WK >= 0) {
+    if (tridiag_wg_supported(pl.n, WK)) {
       TridiagWaveArgs ta;
       ta.H = ga.H;
       ta.DE = ga.DE;
@@ -3127,7 +3130,7 @@ int launch_tridiag_tile(const Plan& pl, GramArgs ga, hipStream_t st) {
       ta.np = ga.np;
       ta.n = pl.n;
       ta.np_dev = ga.np_dev;
-      if (launch_tridiag_wg(ta, F64, st))
+      if (launch_tridiag_wg(ta, WK, st))
         return fail(PDAMD_EHIP, "tridiagonalisation launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
       return PDAMD_OK;

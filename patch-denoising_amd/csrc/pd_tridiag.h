@@ -28,11 +28,13 @@ int tridiag_wave_nb(int n, bool f64);
 int launch_tridiag_wave(const TridiagWaveArgs& a, bool f64, hipStream_t st);
 
 // Workgroup variant (four wavefronts per patch, register-resident 16 x 16
-// tiles) for real Grams up to 16 kWgNT: FP32 above 128, FP64 above 64.
-// Reflector k in row k of the H block (columns k+2 .. n-1) above 128, as the
-// LDS kernel for large Grams writes it; in packed rows at n <= 128.
+// tiles) for Grams up to 16 kWgNT: real FP32 above 128, real FP64 and complex
+// FP32 (Hermitian) above 64.  Reflector k in row k of the H block (columns
+// k+2 .. n-1) above 128, as the LDS kernel for large Grams writes it; in
+// packed rows at n <= 128.
 constexpr int kWgNT = 13;
-int tridiag_wg_supported(int n, bool f64);
-int launch_tridiag_wg(const TridiagWaveArgs& a, bool f64, hipStream_t st);
+enum { kWgF32 = 0, kWgF64 = 1, kWgC32 = 2 };
+int tridiag_wg_supported(int n, int kind);
+int launch_tridiag_wg(const TridiagWaveArgs& a, int kind, hipStream_t st);
 
 }  // namespace pd

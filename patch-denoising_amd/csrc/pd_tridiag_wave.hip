@@ -331,29 +331,70 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE))) vo
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int wg_ld(int nb) { return (nb & 7) == 4 ? nb : wg_ld(nb + 1); }
 
-template <class R>
+// Element arithmetic of the workgroup kernel, real or complex, as explicit
+// FMAs (the same operations wherever a value is recomputed, so that the row
+// and column copies of v, p and w agree bit for bit).
+template <class R> __device__ __forceinline__ R e_fma(R a, R b, R c) { return fma(a, b, c); }
+template <class R> __device__ __forceinline__ cpx<R> e_fma(cpx<R> a, cpx<R> b, cpx<R> c) {  // a b + c
+  return {fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))};
+}
+template <class R> __device__ __forceinline__ R e_fmac(R a, R b, R c) { return fma(a, b, c); }
+template <class R> __device__ __forceinline__ cpx<R> e_fmac(cpx<R> a, cpx<R> b, cpx<R> c) {  // conj(a) b + c
+  return {fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y))};
+}
+// x - v conj(w) - u conj(t)
+template <class R> __device__ __forceinline__ R e_rank2(R x, R v, R w, R u, R t) {
+  return fma(-u, t, fma(-v, w, x));
+}
+template <class R> __device__ __forceinline__ cpx<R> e_rank2(cpx<R> x, cpx<R> v, cpx<R> w, cpx<R> u, cpx<R> t) {
+  const R re = fma(-u.x, t.x, fma(-u.y, t.y, fma(-v.x, w.x, fma(-v.y, w.y, x.x))));
+  const R im = fma(-u.y, t.x, fma(u.x, t.y, fma(-v.y, w.x, fma(v.x, w.y, x.y))));
+  return {re, im};
+}
+template <class R> __device__ __forceinline__ R e_mul(R a, R b) { return a * b; }
+template <class R> __device__ __forceinline__ cpx<R> e_mul(cpx<R> a, cpx<R> b) {
+  return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
+}
+template <class R> __device__ __forceinline__ R e_scale(R a, R s) { return a * s; }
+template <class R> __device__ __forceinline__ cpx<R> e_scale(cpx<R> a, R s) { return {a.x * s, a.y * s}; }
+template <class R> __device__ __forceinline__ R e_sum8_a(R v) { return sum8_a(v); }
+template <class R> __device__ __forceinline__ cpx<R> e_sum8_a(cpx<R> v) { return {sum8_a(v.x), sum8_a(v.y)}; }
+template <class R> __device__ __forceinline__ R e_sum8_b(R v) { return sum8_b(v); }
+template <class R> __device__ __forceinline__ cpx<R> e_sum8_b(cpx<R> v) { return {sum8_b(v.x), sum8_b(v.y)}; }
+template <class R> __device__ __forceinline__ R e_total(R v) { return wave_total(v); }
+template <class R> __device__ __forceinline__ cpx<R> e_total(cpx<R> v) { return {wave_total(v.x), wave_total(v.y)}; }
+template <class R> __device__ __forceinline__ R e_abs2(R a) { return a * a; }
+template <class R> __device__ __forceinline__ R e_abs2(cpx<R> a) { return fma(a.x, a.x, a.y * a.y); }
+
+template <class E>
 struct WgLds {
-  R* X;    // [16][LD]: column k
-  R* RS;   // [2][16][LD]: row parts of p, by wb
-  R* CS;   // [2][16][LD]: column parts of p, by wa
+  using R = typename real_of<E>::type;
+  E* X;    // [16][LD]: column k
+  E* RS;   // [2][16][LD]: row parts of p, by wb
+  E* CS;   // [2][16][LD]: column parts of p, by wa
   R* NRM;  // [2]: |x|^2 parts, by wa
-  R* DOT;  // [4]: v . p parts, by wavefront
-  R* AL;   // [1]: alpha
+  E* DOT;  // [4]: p^H v parts, by wavefront
+  E* AL;   // [1]: alpha
   R* dd;   // [16 NBT]
   R* ee;   // [16 NBT]
-  R* tt;   // [16 NBT]
+  E* tt;   // [16 NBT]
 };
 
 // PK: reflector rows packed (row k at hh_row_w(k, n), the layout of the eigen
 // stage for n <= 128) instead of row k of the n x n block.
-template <class R, int NBT, bool PK, int P>
-__device__ __forceinline__ void wg_phase(R (&L)[NBT * (NBT + 1) / 2], int& k, int last, int n,
-                                         int wa, int wb, int wid, const WgLds<R>& s, R* Hp) {
+// Complex (Hermitian) steps follow ?hetd2: beta real, tau complex, p = A v,
+// w = tau p - (|tau|^2 / 2)(p^H v) v, A -= v w^H + w v^H.
+template <class E, int NBT, bool PK, int P>
+__device__ __forceinline__ void wg_phase(E (&L)[NBT * (NBT + 1) / 2], int& k, int last, int n,
+                                         int wa, int wb, int wid, const WgLds<E>& s, E* Hp) {
+  using R = typename real_of<E>::type;
+  constexpr bool CX = is_cpx<E>::value;
   constexpr int LD = wg_ld(NBT), SL = 16 * LD;
   const int lane = threadIdx.x & (kWave - 1);
   const int a = lane >> 3, b = lane & 7;
   const int ra = 8 * wa + a, cb = 8 * wb + b;  // coordinates inside a tile
   const int kend = min(16 * P + 16, last);
+  const E zero = E(R(0));
   for (; k < kend; ++k) {
     const int u = k & 7;
     // ---- A: column k below the diagonal (wavefronts wb == (k / 8) % 2)
@@ -362,13 +403,13 @@ __device__ __forceinline__ void wg_phase(R (&L)[NBT * (NBT + 1) / 2], int& k, in
 #pragma unroll
       for (int t = P; t < NBT; ++t) {
         const int i = 16 * t + ra;
-        const R x = L[tri_ix(t, P)];
-        const R xv = i > k + 1 ? x : R(0);
-        sq = fma(xv, xv, sq);
+        const E x = L[tri_ix(t, P)];
+        const E xv = i > k + 1 ? x : zero;
+        sq += e_abs2(xv);
         if (b == u) {
           s.X[ra * LD + t] = xv;
           if (i == k + 1) *s.AL = x;
-          if (i == k) s.dd[k] = x;
+          if (i == k) s.dd[k] = re_(x);
         }
       }
       sq = sum8_a(sq);
@@ -377,135 +418,137 @@ __device__ __forceinline__ void wg_phase(R (&L)[NBT * (NBT + 1) / 2], int& k, in
     lds_barrier();
     // ---- B: reflector, v, p = A v
     const R xn2 = s.NRM[0] + s.NRM[1];
-    const R alpha = *s.AL;
-    R beta = alpha, tk = R(0), scal = R(0);
-    if (xn2 > R(0)) {
-      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-      tk = (beta - alpha) / beta;
-      scal = R(1) / (alpha - beta);
+    const E alpha = *s.AL;
+    const R ar = re_(alpha), ai = im_(alpha);
+    R beta = ar;
+    E tk = zero, scal = zero;
+    const bool hh = (xn2 > R(0)) || (ai != R(0));  // H_k != I
+    if (hh) {
+      beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
+      const R ib = R(1) / beta;
+      tk = mk<E>((beta - ar) * ib, -ai * ib);
+      scal = recip_(alpha - E(beta));
     }
-    const R one = xn2 > R(0) ? R(1) : R(0);
+    const E one = E(hh ? R(1) : R(0));
     if (threadIdx.x == 0) {
       s.ee[k] = beta;
       s.tt[k] = tk;
     }
-    R vr[NBT], vc[NBT];
+    E vr[NBT], vc[NBT];
 #pragma unroll
     for (int t = P; t < NBT; ++t) {
-      vr[t] = s.X[ra * LD + t] * scal;
-      vc[t] = s.X[cb * LD + t] * scal;
+      vr[t] = e_mul(s.X[ra * LD + t], scal);
+      vc[t] = e_mul(s.X[cb * LD + t], scal);
       if (t <= P + 1) {
         vr[t] = 16 * t + ra == k + 1 ? one : vr[t];
         vc[t] = 16 * t + cb == k + 1 ? one : vc[t];
       }
     }
     if (wid == (k & 3)) {  // v_k -> row k of H (columns k+2 .. n-1; packed rows also get v_k[k+1])
-      R* row = PK ? Hp + hh_row_w(k, n) - (k + 1) : Hp + (size_t)k * n;
+      E* row = PK ? Hp + hh_row_w(k, n) - (k + 1) : Hp + (size_t)k * n;
       if (PK && lane == 0) row[k + 1] = one;
-      for (int i = k + 2 + lane; i < n; i += kWave) row[i] = s.X[(i & 15) * LD + (i >> 4)] * scal;
+      for (int i = k + 2 + lane; i < n; i += kWave) row[i] = e_mul(s.X[(i & 15) * LD + (i >> 4)], scal);
     }
-    R pc[NBT];
-    R drow = R(0), dcol = R(0);
+    E pc[NBT];
+    E drow = zero, dcol = zero;
 #pragma unroll
     for (int t = P; t < NBT; ++t) {
-      R pr = L[tri_ix(t, t)] * vc[t];  // diagonal tile: row side
+      E pr = e_mul(L[tri_ix(t, t)], vc[t]);  // diagonal tile: row side
 #pragma unroll
       for (int q = P; q < t; ++q) {
-        const R x = L[tri_ix(t, q)];
-        pr = fma(x, vc[q], pr);
-        pc[q] = q + 1 == t ? x * vr[t] : fma(x, vr[t], pc[q]);  // first term of column q at t = q + 1
+        const E x = L[tri_ix(t, q)];
+        pr = e_fma(x, vc[q], pr);
+        // column side: p_j += conj(A_ij) v_i; first term of column q at t = q + 1
+        pc[q] = e_fmac(x, vr[t], q + 1 == t ? zero : pc[q]);
       }
-      pr = sum8_b(pr);
+      pr = e_sum8_b(pr);
       if (b == 0) s.RS[wb * SL + ra * LD + t] = pr;
-      drow = fma(pr, vr[t], drow);
+      drow = e_fmac(pr, vr[t], drow);
     }
 #pragma unroll
     for (int q = P; q + 1 < NBT; ++q) {
-      const R c = sum8_a(pc[q]);
+      const E c = e_sum8_a(pc[q]);
       if (a == 0) s.CS[wa * SL + cb * LD + q] = c;
-      dcol = fma(c, vc[q], dcol);
+      dcol = e_fmac(c, vc[q], dcol);
     }
-    if (a == 0) s.CS[wa * SL + cb * LD + NBT - 1] = R(0);
-    R dp = (b == 0 ? drow : R(0)) + (a == 0 ? dcol : R(0));
-    dp = wave_total(dp);
+    if (a == 0) s.CS[wa * SL + cb * LD + NBT - 1] = zero;
+    E dp = (b == 0 ? drow : zero) + (a == 0 ? dcol : zero);
+    dp = e_total(dp);
     if (lane == 0) s.DOT[wid] = dp;
     lds_barrier();
     // ---- C: w, rank-2 update
-    const R dsum = (s.DOT[0] + s.DOT[1]) + (s.DOT[2] + s.DOT[3]);
-    const R kc = R(-0.5) * tk * (tk * dsum);
+    const E dsum = (s.DOT[0] + s.DOT[1]) + (s.DOT[2] + s.DOT[3]);
+    const E kc = e_scale(dsum, R(-0.5) * e_abs2(tk));
     auto p_at = [&](int o) { return ((s.RS[o] + s.RS[SL + o]) + s.CS[o]) + s.CS[SL + o]; };
-    R wr[NBT];
+    E wr[NBT];
 #pragma unroll
     for (int t = P; t < NBT; ++t) {
-      wr[t] = fma(kc, vr[t], tk * p_at(ra * LD + t));
-      if (t == P) wr[t] = 16 * t + ra > k ? wr[t] : R(0);
+      wr[t] = e_fma(kc, vr[t], e_mul(tk, p_at(ra * LD + t)));
+      if (t == P) wr[t] = 16 * t + ra > k ? wr[t] : zero;
     }
 #pragma unroll
     for (int q = P; q < NBT; ++q) {
-      R wc = fma(kc, vc[q], tk * p_at(cb * LD + q));
-      if (q == P) wc = 16 * q + cb > k ? wc : R(0);
+      E wc = e_fma(kc, vc[q], e_mul(tk, p_at(cb * LD + q)));
+      if (q == P) wc = 16 * q + cb > k ? wc : zero;
 #pragma unroll
-      for (int t = q; t < NBT; ++t) {
-        R x = L[tri_ix(t, q)];
-        x = fma(-vr[t], wc, x);
-        x = fma(-wr[t], vc[q], x);
-        L[tri_ix(t, q)] = x;
-      }
+      for (int t = q; t < NBT; ++t) L[tri_ix(t, q)] = e_rank2(L[tri_ix(t, q)], vr[t], wc, wr[t], vc[q]);
     }
+    (void)CX;
   }
 }
 
-template <class R, int NBT, bool PK, int... Ps>
-__device__ __forceinline__ void wg_phases(std::integer_sequence<int, Ps...>, R (&L)[NBT * (NBT + 1) / 2],
+template <class E, int NBT, bool PK, int... Ps>
+__device__ __forceinline__ void wg_phases(std::integer_sequence<int, Ps...>, E (&L)[NBT * (NBT + 1) / 2],
                                           int& k, int last, int n, int wa, int wb, int wid,
-                                          const WgLds<R>& s, R* Hp) {
-  (wg_phase<R, NBT, PK, Ps>(L, k, last, n, wa, wb, wid, s, Hp), ...);
+                                          const WgLds<E>& s, E* Hp) {
+  (wg_phase<E, NBT, PK, Ps>(L, k, last, n, wa, wb, wid, s, Hp), ...);
 }
 
-template <class R, int NBT, bool PK>
+template <class E, int NBT, bool PK>
 __global__ __launch_bounds__(4 * kWave) void tridiag_wg_kernel(TridiagWaveArgs args) {
+  using R = typename real_of<E>::type;
   constexpr int NT = NBT * (NBT + 1) / 2, LD = wg_ld(NBT), SL = 16 * LD;
   const int n = args.n;
   const int lane = threadIdx.x & (kWave - 1);
   const int a = lane >> 3, b = lane & 7;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int wa = wid >> 1, wb = wid & 1;
-  __shared__ R xs[SL], rs[2 * SL], cs[2 * SL], nrm[2], dot[4], al[1];
-  __shared__ R dd[16 * NBT], ee[16 * NBT], tt[16 * NBT];
-  const WgLds<R> s{xs, rs, cs, nrm, dot, al, dd, ee, tt};
+  __shared__ E xs[SL], rs[2 * SL], cs[2 * SL], dot[4], al[1], tt[16 * NBT];
+  __shared__ R nrm[2], dd[16 * NBT], ee[16 * NBT];
+  const WgLds<E> s{xs, rs, cs, nrm, dot, al, dd, ee, tt};
 
   const int p = blockIdx.x;
   if (args.np_dev && p >= *args.np_dev) return;
-  R* Hp = reinterpret_cast<R*>(args.H) + (size_t)p * n * n;
-  R L[NT];
+  E* Hp = reinterpret_cast<E*>(args.H) + (size_t)p * n * n;
+  E L[NT];
 #pragma unroll
   for (int t = 0; t < NBT; ++t)
 #pragma unroll
     for (int q = 0; q <= t; ++q) {
       const int i = 16 * t + 8 * wa + a, c = 16 * q + 8 * wb + b;
-      L[tri_ix(t, q)] = (i < n && c < n) ? Hp[(size_t)i * n + c] : R(0);
+      L[tri_ix(t, q)] = (i < n && c < n) ? Hp[(size_t)i * n + c] : E(R(0));
     }
   // every wavefront's Gram loads have returned before reflector rows overwrite H
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
 
   int k = 0;
-  wg_phases<R, NBT, PK>(std::make_integer_sequence<int, NBT>{}, L, k, n - 1, n, wa, wb, wid, s, Hp);
+  wg_phases<E, NBT, PK>(std::make_integer_sequence<int, NBT>{}, L, k, n - 1, n, wa, wb, wid, s, Hp);
   {  // d_{n-1}: diagonal tile (n-1) / 16, element ((n-1) % 16, (n-1) % 16)
     const int tl = (n - 1) >> 4, cl = (n - 1) & 15;
 #pragma unroll
     for (int t = 0; t < NBT; ++t)
-      if (t == tl && wa == (cl >> 3) && wb == (cl >> 3) && lane == 9 * (cl & 7)) dd[n - 1] = L[tri_ix(t, t)];
+      if (t == tl && wa == (cl >> 3) && wb == (cl >> 3) && lane == 9 * (cl & 7)) dd[n - 1] = re_(L[tri_ix(t, t)]);
   }
   __syncthreads();
   R* DEp = reinterpret_cast<R*>(args.DE) + (size_t)p * 3 * n;
-  R* TAUp = reinterpret_cast<R*>(args.TAU) + (size_t)p * n;
+  E* TAUp = reinterpret_cast<E*>(args.TAU) + (size_t)p * n;
   for (int q = threadIdx.x; q < n; q += 4 * kWave) {
     const R ev = q < n - 1 ? ee[q] : R(0);
     DEp[q] = dd[q];
     DEp[n + q] = ev;
     DEp[2 * n + q] = ev * ev;
-    TAUp[q] = q < n - 1 ? tt[q] : R(0);
+    TAUp[q] = q < n - 1 ? tt[q] : E(R(0));
   }
 }
 
@@ -522,23 +565,27 @@ int launch_nb(const TridiagWaveArgs& a, hipStream_t st) {
 
 namespace pd {
 
-int tridiag_wg_supported(int n, bool f64) { return n > (f64 ? 64 : 128) && n <= 16 * kWgNT; }
+int tridiag_wg_supported(int n, int kind) {
+  return n > (kind == kWgF32 ? 128 : 64) && n <= 16 * kWgNT;
+}
 
-int launch_tridiag_wg(const TridiagWaveArgs& a, bool f64, hipStream_t st) {
-  if (!tridiag_wg_supported(a.n, f64)) return -2;
-  if (a.np < 1) return 0;
+template <class E>
+static void launch_wg(const TridiagWaveArgs& a, hipStream_t st) {
   // tiles per side: the smallest instantiation that holds the Gram
-  if (f64) {
-    auto kern = a.n <= 128   ? tridiag_wg_kernel<double, 8, true>
-                : a.n <= 144 ? tridiag_wg_kernel<double, 9, false>
-                : a.n <= 176 ? tridiag_wg_kernel<double, 11, false>
-                             : tridiag_wg_kernel<double, kWgNT, false>;
-    kern<<<a.np, 4 * kWave, 0, st>>>(a);
-  } else {
-    auto kern = a.n <= 144   ? tridiag_wg_kernel<float, 9, false>
-                : a.n <= 176 ? tridiag_wg_kernel<float, 11, false>
-                             : tridiag_wg_kernel<float, kWgNT, false>;
-    kern<<<a.np, 4 * kWave, 0, st>>>(a);
+  auto kern = a.n <= 128   ? tridiag_wg_kernel<E, 8, true>
+              : a.n <= 144 ? tridiag_wg_kernel<E, 9, false>
+              : a.n <= 176 ? tridiag_wg_kernel<E, 11, false>
+                           : tridiag_wg_kernel<E, kWgNT, false>;
+  kern<<<a.np, 4 * kWave, 0, st>>>(a);
+}
+
+int launch_tridiag_wg(const TridiagWaveArgs& a, int kind, hipStream_t st) {
+  if (!tridiag_wg_supported(a.n, kind)) return -2;
+  if (a.np < 1) return 0;
+  switch (kind) {
+    case kWgF32: launch_wg<float>(a, st); break;
+    case kWgF64: launch_wg<double>(a, st); break;
+    default: launch_wg<cpx<float>>(a, st); break;
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

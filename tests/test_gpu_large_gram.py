@@ -1,4 +1,5 @@
-"""GPU parity for Gram sizes n = min(patch voxels, frames) above 128.
+"""GPU parity for Gram sizes n = min(patch voxels, frames) above 128 (and a
+complex64 Gram of 100, on the same register-tile Householder kernel).
 
 Patch shapes of the full-size workloads whose Gram exceeds 128 -- 11^3 x 200
 complex NORDIC patches, 7^3-voxel patches of 200 to 400 frames, 6^3 x 300,
@@ -35,6 +36,15 @@ def _lowrank(shape, T, rank, seed, dtype=np.float32, sigma=30.0, base=800.0, amp
     return (clean + sigma * rng.standard_normal(shape + (T,))).astype(dtype)
 
 
+def _lowrank_c(shape, T, rank, seed, sigma=30.0, base=800.0, amp=60.0):
+    """complex64 low-rank volume of the same dynamic range as ``_lowrank``
+    (complex maps and time courses, circular complex noise)."""
+    rng = np.random.default_rng(seed)
+    cn = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    clean = base + amp * np.tensordot(cn(rank, *shape), cn(rank, T), axes=([0], [0])) / 2.0
+    return (clean + sigma / np.sqrt(2.0) * cn(*shape, T)).astype(np.complex64)
+
+
 def _sigma_map(shape, s0=30.0):
     grids = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
     rad = np.sqrt(sum((g - (s - 1) / 2.0) ** 2 for g, s in zip(grids, shape)))
@@ -65,6 +75,12 @@ def _cases():
               (7, 6), {"loss": "fro"}))
     c.append(("mppca_f64_p7_T170", "mp_pca", _lowrank((8, 8, 8), 170, 4, 17, dtype=np.float64),
               (7, 6), {}))
+    # complex64 MP-PCA (complex FP32 eigen stage, Hermitian reduction): n = 100
+    # (packed reflector rows) and n = 200
+    c.append(("mppca_c64_p5_T100", "mp_pca", _lowrank_c((8, 8, 7), 100, 4, 18), (5, 4), {}))
+    c.append(("mppca_c64_p7_T200", "mp_pca", _lowrank_c((9, 8, 8), 200, 5, 19), (7, 6), {}))
+    c.append(("hybrid_c64_p7_T160", "hybrid_pca", _lowrank_c((8, 8, 8), 160, 4, 20), (7, 6),
+              {"noise_std": 30.0}))
     # M-side, n = 216 (6^3 voxels < 260 frames), MP-PCA float64
     c.append(("mppca_f64_p6_T260", "mp_pca",
               _lowrank((8, 8, 7), 260, 5, 2, dtype=np.float64), (6, 5), {}))
@@ -121,7 +137,7 @@ def test_large_gram_matches_oracle(case):
     name, fn, data, (ps, po), kw = case
     vox = int(np.prod(ps)) if isinstance(ps, tuple) else ps ** (data.ndim - 1)
     n = min(vox, data.shape[-1])
-    assert n > 128
+    assert n > 64
     print(f"[large-gram] {name}: shape {data.shape} {data.dtype} patch {ps} overlap {po} n {n}",
           flush=True)
     np.random.seed(0)
