@@ -1960,28 +1960,63 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
         dv = guard((d[i + 1] - lm) - l * e[i]);
         dp[(size_t)(i + 1) * ks] = dv;
       }
+      // the loops below read the scratch columns back in blocks of 8 fetched
+      // ahead of the recurrence, so it waits on memory once per block
       R dm = guard(d[n - 1] - lm);
       R best = fabs(dp[(size_t)(n - 1) * ks] + dm - (d[n - 1] - lm));
       int r = n - 1;
-      for (int i = n - 2; i >= 0; --i) {
+      auto bottom_up = [&](int i, R dpi) {
         const R u = qdiv_(e[i], dm);
         um[(size_t)(i + 1) * ks] = u;
         dm = guard((d[i] - lm) - u * e[i]);
-        const R g = fabs(dp[(size_t)i * ks] + dm - (d[i] - lm));
+        const R g = fabs(dpi + dm - (d[i] - lm));
         if (g < best) {
           best = g;
           r = i;
         }
+      };
+      int i = n - 2;
+      for (; i >= 7; i -= 8) {
+        R pf[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pf[q] = dp[(size_t)(i - q) * ks];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bottom_up(i - q, pf[q]);
       }
+      for (; i >= 0; --i) bottom_up(i, dp[(size_t)i * ks]);
       z[(size_t)r * ks] = R(1);
       R zi = R(1), ss = R(1);
-      for (int i = r - 1; i >= 0; --i) {
+      i = r - 1;
+      for (; i >= 7; i -= 8) {
+        R pf[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pf[q] = dp[(size_t)(i - q) * ks];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          zi = -qdiv_(e[i - q], pf[q]) * zi;
+          z[(size_t)(i - q) * ks] = zi;
+          ss += zi * zi;
+        }
+      }
+      for (; i >= 0; --i) {
         zi = -qdiv_(e[i], dp[(size_t)i * ks]) * zi;
         z[(size_t)i * ks] = zi;
         ss += zi * zi;
       }
       zi = R(1);
-      for (int i = r + 1; i < n; ++i) {
+      i = r + 1;
+      for (; i + 8 <= n; i += 8) {
+        R pf[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pf[q] = um[(size_t)(i + q) * ks];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          zi = -pf[q] * zi;
+          z[(size_t)(i + q) * ks] = zi;
+          ss += zi * zi;
+        }
+      }
+      for (; i < n; ++i) {
         zi = -um[(size_t)i * ks] * zi;
         z[(size_t)i * ks] = zi;
         ss += zi * zi;
@@ -2004,7 +2039,15 @@ __device__ void all_pairs_vectors(const R* __restrict__ d, const R* __restrict__
         for (int i = 0; i < n; ++i) ss += z[(size_t)i * ks] * z[(size_t)i * ks];
       }
       const R inv = R(1) / sqrt(fmax(ss, Lim<R>::tiny()));
-      for (int i = 0; i < n; ++i) z[(size_t)i * ks] *= inv;
+      i = 0;
+      for (; i + 8 <= n; i += 8) {
+        R zv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) zv[q] = z[(size_t)(i + q) * ks];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) z[(size_t)(i + q) * ks] = zv[q] * inv;
+      }
+      for (; i < n; ++i) z[(size_t)i * ks] *= inv;
     }
   }
   __syncthreads();  // columns written lane-wise, read row-wise below
