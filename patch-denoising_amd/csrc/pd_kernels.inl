@@ -1682,6 +1682,70 @@ __device__ void multisect(WaveScratch<R>& s, const R* __restrict__ d, const R* _
   }
 }
 
+// Eigenvalues at ascending indices tgt[0..nt), 64 < nt <= 128: plain
+// bisection (the multisection above with one lane per target), lane j
+// bisecting targets j and j + 64 in the same pass: two independent Sturm
+// recurrences per lane hide each other's latency; the sequence of midpoints
+// and the results are those of two passes of 64.
+template <class R, bool QD = false>
+__device__ void bisect_pairs(WaveScratch<R>& s, const R* __restrict__ d, const R* __restrict__ e2,
+                             int n, R pivmin, R glo, R ghi, R atol, int nt) {
+  const int lane = threadIdx.x;
+  const bool a0 = lane < nt, a1 = lane + kWave < nt;
+  const int t0 = a0 ? s.tgt[lane] : 0, t1 = a1 ? s.tgt[lane + kWave] : 0;
+  R lo0 = glo, hi0 = ghi, lo1 = glo, hi1 = ghi;
+  auto div = [](R a, R b) { return QD ? qdiv_(a, b) : fdiv_(a, b); };
+  for (int it = 0; it < 200; ++it) {
+    const R tol0 = atol + R(2) * Lim<R>::ulp() * fmax(fabs(lo0), fabs(hi0));
+    const R tol1 = atol + R(2) * Lim<R>::ulp() * fmax(fabs(lo1), fabs(hi1));
+    const R x0 = lo0 + (hi0 - lo0) * R(0.5), x1 = lo1 + (hi1 - lo1) * R(0.5);
+    const bool done0 = !a0 || (hi0 - lo0) <= tol0 || x0 <= lo0 || x0 >= hi0;
+    const bool done1 = !a1 || (hi1 - lo1) <= tol1 || x1 <= lo1 || x1 >= hi1;
+    if (__all(done0 && done1)) break;
+    R q0 = d[0] - x0, q1 = d[0] - x1;
+    if (fabs(q0) < pivmin) q0 = -pivmin;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    int c0 = q0 < R(0), c1 = q1 < R(0);
+    int i = 1;
+    for (; i + 8 <= n; i += 8) {
+      R dv[8], ev[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        dv[u] = d[i + u];
+        ev[u] = e2[i - 1 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        q0 = (dv[u] - x0) - div(ev[u], q0);
+        q1 = (dv[u] - x1) - div(ev[u], q1);
+        if (fabs(q0) < pivmin) q0 = -pivmin;
+        if (fabs(q1) < pivmin) q1 = -pivmin;
+        c0 += q0 < R(0);
+        c1 += q1 < R(0);
+      }
+    }
+    for (; i < n; ++i) {
+      q0 = (d[i] - x0) - div(e2[i - 1], q0);
+      q1 = (d[i] - x1) - div(e2[i - 1], q1);
+      if (fabs(q0) < pivmin) q0 = -pivmin;
+      if (fabs(q1) < pivmin) q1 = -pivmin;
+      c0 += q0 < R(0);
+      c1 += q1 < R(0);
+    }
+    if (!done0) {
+      if (c0 <= t0) lo0 = x0;
+      else hi0 = x0;
+    }
+    if (!done1) {
+      if (c1 <= t1) lo1 = x1;
+      else hi1 = x1;
+    }
+  }
+  if (a0) s.val[lane] = lo0 + (hi0 - lo0) * R(0.5);
+  if (a1) s.val[lane + kWave] = lo1 + (hi1 - lo1) * R(0.5);
+  wave_sync();
+}
+
 // Solve (T - lam I) x = b for the tridiagonal (d, e) with partial pivoting.
 // d / e are wave-uniform global arrays (e[n-1] = 0); b is in rhs (stride ks)
 // and the solution overwrites it; u0 / u1 / u2 (stride ks) hold U.
@@ -2708,7 +2772,10 @@ __global__ __launch_bounds__(kWave) void trieig_all_kernel(EigArgs a) {
     ghi += R(4) * Lim<R>::ulp() * tnorm + R(4) * pivmin;
     for (int j = lane; j < n; j += kWave) s.tgt[j] = j;
     wave_sync();
-    multisect<R, true>(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
+    if (n > kWave && n <= 2 * kWave)
+      bisect_pairs<R, true>(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
+    else
+      multisect<R, true>(s, dg, e2g, n, pivmin, glo, ghi, atol, n);
     for (int c = lane; c < n; c += kWave) a.WC[(size_t)p * a.kmax + c] = (double)s.val[c];
     const R eps_piv = Lim<R>::ulp() * tnorm + pivmin;
     const R clus = (sizeof(R) == 8 ? R(1e-8) : R(1e-3)) * tnorm;
