@@ -856,6 +856,8 @@ __device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v) {
 // BF: split-BF16 three-product Gram (each centred value x = hi + lo, hi =
 // bf16(x), lo = bf16(x - hi); sum hi hi + hi lo + lo hi on MFMA 32x32x16 bf16
 // with FP32 accumulation; complex as four real products), FP32 data only.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
 template <class IN, class E, bool TSIDE, bool BF = false>
 __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vol, GramArgs a) {
   using R = typename real_of<E>::type;
@@ -865,6 +867,7 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
   constexpr int LW = kGBT + 1;
   constexpr bool MF = std::is_same<R, float>::value;  // FP32 real / complex on MFMA
+  constexpr bool MD = std::is_same<R, double>::value;  // FP64 real / complex on MFMA 16x16x4
   constexpr bool CPX = is_cpx<E>::value;
   constexpr bool BFM = BF && MF;
   constexpr int NPL = CPX ? 2 : 1;  // real planes per operand (re, im)
@@ -946,9 +949,20 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
       while ((bi + 1) * (bi + 2) / 2 <= tp) ++bi;
       const int bj = tp - bi * (bi + 1) / 2;
       const int i0 = bi * kGBT, j0 = bj * kGBT;
-      E acc[MF ? 1 : 4][MF ? 1 : 4];
+      E acc[MF || MD ? 1 : 4][MF || MD ? 1 : 4];
       floatx16 cre, cim;
-      if constexpr (MF) {
+      // FP64: wave (wi, wj) computes the 32 x 32 quadrant of the 64 x 64 tile
+      // as 2 x 2 MFMA tiles of 16 x 16 (k in steps of 4; lane l supplies
+      // A[16 ta + l % 16][k + l / 16] and B[k + l / 16][16 tb + l % 16]);
+      // quadrants past n or above the diagonal are not computed
+      doublex4 dre[2][2], dim[2][2];
+      const bool qlive = i0 + 32 * wi < n && j0 + 32 * wj < n && !(bi == bj && wj > wi);
+      if constexpr (MD) {
+#pragma unroll
+        for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+          for (int tb = 0; tb < 2; ++tb) dre[ta][tb] = dim[ta][tb] = doublex4{0.0, 0.0, 0.0, 0.0};
+      } else if constexpr (MF) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) cre[q] = cim[q] = 0.0f;
       } else {
@@ -957,7 +971,73 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
 #pragma unroll
           for (int v = 0; v < 4; ++v) acc[u][v] = E(R(0));
       }
-      for (int k0 = 0; k0 < K; k0 += kGBK) {
+      if constexpr (MD) {
+        // the next chunk's centred values are loaded into registers while the
+        // current chunk is multiplied
+        constexpr int NE = kGBK * kGBT / kNT;
+        IN ra[NE], rb[NE];
+        auto fetch = [&](IN* rr, int b0, int k0, int kc) {
+#pragma unroll
+          for (int u = 0; u < NE; ++u) {
+            const int q = tid + u * kNT;
+            const int i = TSIDE ? q % kGBT : q / kGBK, k = TSIDE ? q / kGBT : q % kGBK;
+            const int row = TSIDE ? k0 + k : b0 + i, tt = TSIDE ? b0 + i : k0 + k;
+            rr[u] = (k < kc && b0 + i < n) ? IN(X[roff[row] + tt] - mean[tt]) : IN(0);
+          }
+        };
+        auto stash = [&](E* S, const IN* rr) {
+#pragma unroll
+          for (int u = 0; u < NE; ++u) {
+            const int q = tid + u * kNT;
+            const int i = TSIDE ? q % kGBT : q / kGBK, k = TSIDE ? q / kGBT : q % kGBK;
+            S[k * LW + i] = cvt<E>(rr[u]);
+          }
+        };
+        fetch(ra, i0, 0, min(kGBK, K));
+        fetch(rb, j0, 0, min(kGBK, K));
+        const int l16 = lane & 15, l4 = lane >> 4;
+        for (int k0 = 0; k0 < K; k0 += kGBK) {
+          const int kc = min(kGBK, K - k0);
+          stash(As, ra);
+          stash(Bs, rb);
+          lds_barrier();
+          if (k0 + kGBK < K) {
+            fetch(ra, i0, k0 + kGBK, min(kGBK, K - k0 - kGBK));
+            fetch(rb, j0, k0 + kGBK, min(kGBK, K - k0 - kGBK));
+          }
+          // rows k >= kc of the staging are zero
+          if (qlive) {
+            for (int k = 0; k < kc; k += 4) {
+              E fa[2], fb[2];
+#pragma unroll
+              for (int t = 0; t < 2; ++t) {
+                fa[t] = As[(k + l4) * LW + 32 * wi + 16 * t + l16];
+                fb[t] = Bs[(k + l4) * LW + 32 * wj + 16 * t + l16];
+              }
+#pragma unroll
+              for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb) {
+                  const double ar = re_(fa[ta]), br = re_(fb[tb]);
+                  dre[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, dre[ta][tb], 0, 0, 0);
+                  if constexpr (CPX) {
+                    const double ai = im_(fa[ta]), bim = im_(fb[tb]);
+                    dre[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bim, dre[ta][tb], 0, 0, 0);
+                    if (TSIDE) {  // conj(a) b: im = ar bi - ai br
+                      dim[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bim, dim[ta][tb], 0, 0, 0);
+                      dim[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, dim[ta][tb], 0, 0, 0);
+                    } else {  // a conj(b): im = ai br - ar bi
+                      dim[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, dim[ta][tb], 0, 0, 0);
+                      dim[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, bim, dim[ta][tb], 0, 0, 0);
+                    }
+                  }
+                }
+            }
+          }
+          lds_barrier();
+        }
+      }
+      for (int k0 = 0; k0 < (MD ? 0 : K); k0 += kGBK) {
         const int kc = min(kGBK, K - k0);
         if constexpr (BFM) {
           stage_bf(0, i0, k0, kc);
@@ -1037,7 +1117,28 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
         }
         lds_barrier();
       }
-      if constexpr (MF) {
+      if constexpr (MD) {
+        // 16x16 MFMA output: accumulator q of lane l is C[4 q + l / 16][l % 16]; the
+        // lower quadrant of a diagonal tile also writes its mirror
+        if (qlive) {
+          const int l16 = lane & 15, l4 = lane >> 4;
+#pragma unroll
+          for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int r = i0 + 32 * wi + 16 * ta + 4 * q + l4;
+                const int c = j0 + 32 * wj + 16 * tb + l16;
+                if (r < n && c < n) {
+                  E x = mk<E>(dre[ta][tb][q], CPX ? dim[ta][tb][q] : 0.0);
+                  if (a.scale) x = div_n(x, M - 1);
+                  G[(size_t)r * n + c] = x;
+                  if (bi != bj || wi != wj) G[(size_t)c * n + r] = conj_(x);
+                }
+              }
+        }
+      } else if constexpr (MF) {
         // 32x32 MFMA output: accumulator q of lane l is C[8(q/4) + 4(l/32) + q%4][l%32]
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
