@@ -111,23 +111,25 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
     pl->kmax = n;
   }
   pl->kb = std::min(pl->tile ? kKB : (n > kNMid ? 1 : kKBBig), pl->kmax);
-  // recombination by gather: M-side float32 patches, one temporal patch,
-  // weighted / average, the eigen rules (10 components), not in a workspace
-  // of capped size (the per-call factor buffers would not fit)
+  // recombination by gather: float32 patches (M-side Grams up to 128,
+  // T-side up to 256 over at most 512 frames), one temporal patch, weighted /
+  // average, the eigen rules (10 components), not in a workspace of capped
+  // size (the per-call factor buffers would not fit)
   if (P->recombine_mode < 0 || P->recombine_mode > 2)
     return fail(PDAMD_EINVAL, "bad recombine_mode %d", P->recombine_mode);
   {
-    const bool elig = !bwd && P->dtype == PDAMD_F32 && pl->etype == 0 && !pl->tside &&
-                      g.gt == 1 &&
+    const bool elig = !bwd && P->dtype == PDAMD_F32 && pl->etype == 0 && g.gt == 1 &&
                       (P->recombination == PDAMD_WEIGHTED || P->recombination == PDAMD_AVERAGE) &&
                       (P->method == PDAMD_MPPCA || P->method == PDAMD_HYBRID) &&
-                      pl->kmax == kGatherKM && n <= 128 &&
+                      pl->kmax == kGatherKM &&
+                      (pl->tside ? (n <= 256 && g.pt <= 512) : n <= 128) &&
                       pl->nvox * g.T < ((int64_t)1 << 31);
     if (P->recombine_mode == PDAMD_RECOMBINE_GATHER && (!elig || chunk_cap > 0))
       return fail(PDAMD_EUNSUPPORTED,
-                  "gather recombination needs float32 data, one temporal patch, more frames "
-                  "than patch voxels, weighted or average recombination, the MP-PCA or Hybrid "
-                  "rule and an uncapped workspace");
+                  "gather recombination needs float32 data, one temporal patch, weighted or "
+                  "average recombination, the MP-PCA or Hybrid rule, a Gram of at most 128 "
+                  "(voxel side) or 256 (frame side, at most 512 frames) and an uncapped "
+                  "workspace");
     pl->gather = elig && P->recombine_mode != PDAMD_RECOMBINE_SCATTER && chunk_cap == 0;
   }
   // FP64 re-check of near-threshold decisions: float32 data whose eigen
@@ -176,7 +178,8 @@ int make_plan(const pdamd_problem* P, Plan* pl, bool bwd = false, int64_t chunk_
     pl->off_GNC = take(NPs * sizeof(int));
     pl->off_GTH = take(NPs * sizeof(double));
     pl->off_GMEAN = take(NPs * pt * sizeof(float));
-    pl->off_GY = take(NPs * n * kGatherKM * sizeof(float));
+    pl->off_GY = take(NPs * M * kGatherKM * sizeof(float));
+    if (pl->tside) pl->off_GV = take(NPs * n * kGatherKM * sizeof(float));
     pl->off_GRF = take(NPs * (kGatherKM + 1) * pt * sizeof(float));
     pl->off_GW3 = take((size_t)pl->nvox * sizeof(float));
   }

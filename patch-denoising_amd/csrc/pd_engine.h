@@ -85,7 +85,7 @@ struct Plan {
   size_t off_cnt, off_flags, off_btot, off_idx, off_count, off_H, off_DE, off_TAU, off_Y, off_Z,
       off_WC, off_NC, off_TH, off_w3, off_SCR, total;
   // gather recombination only (per call, indexed by selected-patch slot)
-  size_t off_GMAP = 0, off_GNC = 0, off_GTH = 0, off_GMEAN = 0, off_GY = 0, off_GRF = 0,
+  size_t off_GMAP = 0, off_GNC = 0, off_GTH = 0, off_GMEAN = 0, off_GY = 0, off_GV = 0, off_GRF = 0,
          off_GW3 = 0;
   // FP64 re-check of near-threshold decisions (float32 data, FP32 eigen
   // stage, MP-PCA / Hybrid): per-patch flags, the list of at most rcap

@@ -1,13 +1,17 @@
 // Gather-based recombination (pd_gather.hip): replaces the atomic
-// scatter-accumulate of the synthesis for M-side float32 patches with at most
-// 10 components (MP-PCA, Hybrid), one temporal patch, weighted / average
+// scatter-accumulate of the synthesis for float32 patches with at most 10
+// components (MP-PCA, Hybrid), one temporal patch, weighted / average
 // recombination.
 //
-// Per selected patch (slot j, patches in increasing linear index):
+// Per selected patch (slot j, patches in increasing linear index), M-side
+// (Gram Xc Xc^T, eigenvectors U):
 //   Lf_j = U_j (M x KM rows of the Gram's kept eigenvectors, zero past the
 //          kept count) -- the eigen stage's per-call eigenvector buffer;
 //   Rf_j = (KM + 1) x T: row 0 = theta_j m_j (per-frame mean), row 1 + k =
 //          theta_j wc_k (U_j^T Xc_j)[k] (zero past the kept count);
+// T-side (Gram Xc^T Xc, eigenvectors V, T x KM in a per-call buffer):
+//   Lf_j = Xc_j V_j diag(wc) (M x KM), Rf_j row 0 = theta_j m_j, row 1 + k
+//          = theta_j V_j[:, k]^T;
 // so that theta_j * (denoised patch j) = Lf_j Rf_j[1:] + 1 Rf_j[0].
 // Every output element is then the sum of its covering patches' rows in
 // increasing patch index (the reference's C-order loop), divided by the
@@ -26,7 +30,7 @@ constexpr int kGatherKM = 10;  // components per patch in the factor layout
 struct GatherArgs {
   Geom g;
   int64_t shape[3];      // spatial sizes
-  int n;                 // Gram size (== M)
+  int n;                 // rows of a left factor (patch voxels M)
   int np;                // patches of this launch (factor kernel: the chunk)
   int slot0;             // factor kernel: slot of the chunk's first patch
   int recomb;            // PDAMD_WEIGHTED / PDAMD_AVERAGE
@@ -34,7 +38,10 @@ struct GatherArgs {
   const int32_t* idx;    // selected linear patch indices (all slots)
   const float* vol;      // input volume
   const float* MEAN;     // slots x T per-frame means
-  const float* Y;        // slots x n x KM eigenvectors (zero past the kept count)
+  const float* Y;        // slots x M x KM left factors Lf (M-side: the eigenvectors)
+  float* YT;             // T-side: the left factors, written by the factor kernel
+  const float* V;        // T-side: slots x T x KM eigenvectors (zero past the kept count)
+  int tside;             // 1: T-side factors
   const double* WC;      // chunk x KM component weights (factor kernel)
   const int* NCOMP;      // slots: kept components
   const double* THETA;   // slots: 1 / (2 + rank)

@@ -100,6 +100,92 @@ __global__ __launch_bounds__(kWave) void gather_factor_kernel(GatherArgs a) {
   }
 }
 
+// T-side factors of one patch per wavefront: V (T x KM) and the means in LDS
+// (read as broadcasts), lanes = voxel rows, 64 per pass, each lane streaming
+// its row's frames (four per load when rows are 16-byte aligned).
+__global__ __launch_bounds__(kWave) void gather_factor_t_kernel(GatherArgs a) {
+  extern __shared__ __align__(16) float fts[];
+  const int p = blockIdx.x;  // chunk-local patch, or list entry
+  if (a.list && p >= *a.list_cnt) return;
+  const int slot = a.list ? a.list[p] : a.slot0 + p;
+  const int lane = threadIdx.x;
+  const Geom& g = a.g;
+  const int M = g.M, T = g.pt;
+  float* vs = fts;            // [T][KM]
+  float* ms = fts + T * KM;   // [T]
+  int64_t sp;
+  int t0;
+  patch_origin(g, a.idx[slot], sp, t0);
+  const float* __restrict__ X = a.vol + sp + t0;
+  const float* __restrict__ V = a.V + (size_t)slot * T * KM;
+  const float* __restrict__ mean = a.MEAN + (size_t)slot * T;
+  const int nc = a.NCOMP[slot];
+  const double theta = a.recomb == PDAMD_WEIGHTED ? a.THETA[slot] : 1.0;
+  const float th = (float)theta;
+  float wk[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k)
+    wk[k] = k < nc ? (float)(a.list ? 1.0 : a.WC[(size_t)p * KM + k]) : 0.f;
+  float* __restrict__ R = a.RF + (size_t)slot * (KM + 1) * T;
+  for (int t = lane; t < T; t += kWave) {
+    const float m = mean[t];
+    ms[t] = m;
+    R[t] = th * m;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const float v = k < nc ? V[(size_t)t * KM + k] : 0.f;
+      vs[t * KM + k] = v;
+      R[(size_t)(1 + k) * T + t] = th * v;
+    }
+  }
+  wave_sync();
+  float* __restrict__ U = a.YT + (size_t)slot * M * KM;
+  const bool v4 = (T & 3) == 0 && ((sp + t0) & 3) == 0 && (g.vs2 & 3) == 0 && (g.vs1 & 3) == 0 &&
+                  (g.vs0 & 3) == 0;
+  for (int rb = 0; rb < M; rb += kWave) {
+    const int r = rb + lane;
+    const bool rv = r < M;
+    int off = 0;
+    if (rv) {
+      const int i2 = r % g.p2, q = r / g.p2;
+      off = (int)((q / g.p1) * g.vs0 + (q % g.p1) * g.vs1 + i2 * g.vs2);
+    }
+    const float* __restrict__ xr = X + off;
+    float acc[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) acc[k] = 0.f;
+    auto add = [&](int t, float x) {
+      const float xc = x - ms[t];
+      const float* vr = vs + t * KM;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) acc[k] = fmaf(xc, vr[k], acc[k]);
+    };
+    int t = 0;
+    if (v4) {
+      for (; t + 4 <= T; t += 4) {
+        const float4 x = *reinterpret_cast<const float4*>(xr + t);
+        add(t, x.x);
+        add(t + 1, x.y);
+        add(t + 2, x.z);
+        add(t + 3, x.w);
+      }
+    } else {
+      for (; t + 4 <= T; t += 4) {
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = xr[t + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) add(t + u, x[u]);
+      }
+    }
+    for (; t < T; ++t) add(t, xr[t]);
+    if (rv) {
+#pragma unroll
+      for (int k = 0; k < KM; ++k) U[(size_t)r * KM + k] = wk[k] * acc[k];
+    }
+  }
+}
+
 __global__ void gather_map_fill_kernel(int32_t* __restrict__ map, int64_t npatch) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npatch;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -345,6 +431,12 @@ namespace pd {
 int launch_gather_factors(const GatherArgs& a, hipStream_t st) {
   const int nb = a.list ? a.list_cap : a.np;
   if (nb < 1) return 0;
+  if (a.tside) {
+    const size_t lds = (size_t)a.g.pt * (KM + 1) * sizeof(float);
+    if (lds > 64 * 1024) return -1;
+    gather_factor_t_kernel<<<nb, kWave, lds, st>>>(a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
   const int nch = (a.g.pt + kWave - 1) / kWave;
   if (nch <= 2) gather_factor_kernel<2><<<nb, kWave, 0, st>>>(a);
   else if (nch <= 3) gather_factor_kernel<3><<<nb, kWave, 0, st>>>(a);

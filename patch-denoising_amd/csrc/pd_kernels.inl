@@ -3720,9 +3720,13 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
   fa.shape[0] = P->shape[0];
   fa.shape[1] = P->shape[1];
   fa.shape[2] = P->shape[2];
-  fa.n = pl.n;
+  const bool ts = pl.tside;
+  fa.n = pl.g.M;
   fa.np = 0;
   fa.slot0 = 0;
+  fa.tside = ts ? 1 : 0;
+  fa.YT = ts ? reinterpret_cast<float*>(ws + pl.off_GY) : nullptr;
+  fa.V = ts ? reinterpret_cast<const float*>(ws + pl.off_GV) : nullptr;
   fa.recomb = P->recombination;
   fa.theta_weak = P->method == PDAMD_HYBRID ? 1 : 0;
   fa.idx = idx_all;
@@ -3743,9 +3747,14 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
   for (int64_t c0 = 0; c0 < count; c0 += chunk) {
     const int np = (int)std::min<int64_t>(chunk, count - c0);
     GramArgs ga = gram_args(P, pl, ws, idx_all + c0, np);
-    ga.MEANW = ws + pl.off_GMEAN + (size_t)c0 * T * sizeof(float);
+    // the per-frame means straight into the per-call buffer (T-side Grams
+    // write them as the synthesis input, M-side ones as the gather's)
+    void* mean_c = ws + pl.off_GMEAN + (size_t)c0 * T * sizeof(float);
+    if (ts) ga.MEAN = mean_c;
+    else ga.MEANW = mean_c;
     tm.begin(PDAMD_STAGE_GRAM);
-    int rc = launch_gram_tile<float, float, false>(pl, P->input_dev, ga, st);
+    int rc = ts ? launch_gram_tile<float, float, true>(pl, P->input_dev, ga, st)
+                : launch_gram_tile<float, float, false>(pl, P->input_dev, ga, st);
     tm.end();
     if (rc) return rc;
     tm.begin(PDAMD_STAGE_TRIDIAG);
@@ -3753,7 +3762,7 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
     tm.end();
     if (rc) return rc;
     EigArgs ea = eig_args(P, pl, ws, ga);
-    ea.Y = ws + pl.off_GY + (size_t)c0 * pl.n * kGatherKM * sizeof(float);
+    ea.Y = ws + (ts ? pl.off_GV : pl.off_GY) + (size_t)c0 * pl.n * kGatherKM * sizeof(float);
     ea.NCOMP = reinterpret_cast<int*>(ws + pl.off_GNC) + c0;
     ea.THETA = reinterpret_cast<double*>(ws + pl.off_GTH) + c0;
     ea.zero_fill = 1;
@@ -3780,7 +3789,7 @@ int run_chunks_gather(pdamd_problem* P, const Plan& pl, char* ws, int64_t count,
     // the call's near-threshold patches in FP64, then their factors again
     RefineStore sto{};
     sto.kmax_dst = kGatherKM;
-    sto.Ydst = reinterpret_cast<float*>(ws + pl.off_GY);
+    sto.Ydst = reinterpret_cast<float*>(ws + (ts ? pl.off_GV : pl.off_GY));
     sto.NCdst = reinterpret_cast<int*>(ws + pl.off_GNC);
     sto.THdst = reinterpret_cast<double*>(ws + pl.off_GTH);
     sto.WCdst = nullptr;

@@ -10,6 +10,12 @@ increasing patch index, no atomics) against the atomic scatter and the oracle.
   output matches it to 1e-5.
 * Hybrid (a Python-float theta: float32 additions) with a mask: weights equal
   the oracle's bit for bit.
+* T-side patches (fewer frames than voxels: left factors Xc V diag(wc), right
+  factors theta [m; V^T]): two gather runs are bit-identical and agree with
+  the scatter to float32 rounding with identical ranks (7^3 x 200, Gram size
+  200; 5^3 x 60, Gram size 60); on the 32x32x16x50 workload with 5^3 patches
+  the weights equal the oracle's bit for bit and the output matches it to
+  1e-5.
 """
 
 import numpy as np
@@ -86,12 +92,53 @@ def test_gather_hybrid_masked_weights_bit_exact():
     assert nbad == 0 and e <= 1e-5
 
 
+def _tside_volume(shape, T, seed, rank=4):
+    rng = np.random.default_rng(seed)
+    sig = 600 + 40 * np.tensordot(rng.standard_normal((rank,) + shape),
+                                  rng.standard_normal((rank, T)), axes=([0], [0]))
+    return (sig + 25 * rng.standard_normal(shape + (T,))).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,T,p", [((11, 12, 10), 200, 7), ((12, 12, 10), 60, 5)],
+                         ids=["p7_T200", "p5_T60"])
+def test_gather_tside_repeats_bitwise_and_matches_scatter(shape, T, p):
+    from patch_denoise import MPPCADenoiser
+
+    data = _tside_volume(shape, T, 7)
+    den = MPPCADenoiser(p, p - 1, threshold_scale=1.0)
+    print(f"[gather] T-side {shape}x{T} p{p}: gather x2, scatter x1", flush=True)
+    g1, w1, i1 = _run(den, data, None, "gather")
+    g2, w2, _ = _run(den, data, None, "gather")
+    s1, ws, is_ = _run(den, data, None, "scatter")
+    assert np.array_equal(g1, g2, equal_nan=True) and np.array_equal(w1, w2)
+    assert np.array_equal(i1["rank"], is_["rank"])
+    e, ew = rel_l2(g1, s1), rel_l2(w1, ws)
+    print(f"[gather] T-side gather vs scatter: out {e:.2e} weights {ew:.2e}", flush=True)
+    assert e <= 1e-6 and ew <= 1e-6
+
+
+def test_gather_tside_weights_bit_exact_against_oracle():
+    from oracle import cpu_ref as O
+    from patch_denoise import MPPCADenoiser, workloads
+
+    data = workloads.config1()
+    out, w, info = _run(MPPCADenoiser(5, 2, threshold_scale=1.0), data, None, "gather")
+    ref, ref_w, _, _ = O.mp_pca(data, 5, 2)
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    nbad = int(np.sum(w != ref_w))
+    e = rel_l2(out, ref)
+    print(f"[gather] T-side 32x32x16x50 p5: weights differing {nbad}, out rel-L2 {e:.2e}",
+          flush=True)
+    assert nbad == 0 and e <= 1e-5
+
+
 def test_gather_refused_where_ineligible():
-    """Forcing the gather on a T-side problem (fewer frames than voxels) is an error."""
-    from patch_denoise import MPPCADenoiser, _native as N
+    """Forcing the gather on a rule other than MP-PCA / Hybrid (raw SVT) is an error."""
+    from patch_denoise import RawSVDDenoiser, _native as N
 
     data = np.random.default_rng(1).standard_normal((12, 12, 10, 60)).astype(np.float32) + 50
-    den = MPPCADenoiser(5, 4, threshold_scale=1.0)
+    den = RawSVDDenoiser(5, 4, threshold_value=3.0)
+    den._threshold = 3.0  # set by denoise() from threshold_value * threshold_scale
     prob, p_s, p_o, mask_arr = den._prepare(data, None, 50)
     den._configure(prob, data, p_s)
     prob.recombine_mode = N.RECOMBINE["gather"]
