@@ -274,8 +274,9 @@ def bench_batched(steps, autotune=True):
     vols = workloads.config4()
     calls, skipped = workloads.config4_calls(vols)
     gen_s = time.perf_counter() - t0
+    # at most two sets of workspaces alive at a time (the single calls and the
+    # eager batch, then the eager batch and the graph)
     eager = BatchPlan(calls, use_graph=False)
-    graph = BatchPlan(calls, use_graph=True)
     singles = [N.DeviceRun(j.prob, j.data, j.mask_arr, j.var_map) for j in eager.jobs]
     vf = float(sum(np.prod(j.data.shape) for j in eager.jobs))
 
@@ -289,7 +290,9 @@ def bench_batched(steps, autotune=True):
         return (time.perf_counter() - t) / steps
 
     t_seq = timed(lambda: [r.run() for r in singles])
+    del singles
     t_eager = timed(eager.run)
+    graph = BatchPlan(calls, use_graph=True)
     t_graph = timed(graph.run)
     del graph
     # error of every volume against the same call on its float64 cast (the

@@ -291,6 +291,26 @@ def covering(sel, grid, ps, po, lo, hi):
     return np.asarray(sel)[keep]
 
 
+def window_weights32(ps, po, grid, ranks, lo, hi, T):
+    """Weights of the window lo..hi summed as the reference sums them: a
+    float32 array, each patch (increasing linear index) adding its theta =
+    1/(2+r) as f32(f64(w) + theta) (space_time/base.py:243-256, with NumPy's
+    float64 scalar theta); ``ranks`` maps every patch meeting the window to
+    its rank (``oracle_window``'s third result)."""
+    step = [p - o for p, o in zip(ps, po)]
+    ws = tuple(h - l for l, h in zip(lo, hi))
+    w = np.zeros(ws + (T,), np.float32)
+    for L in sorted(ranks):
+        g = np.unravel_index(int(L), grid)
+        o = [gi * st for gi, st in zip(g[:3], step[:3])]
+        a = [max(lo[d], o[d]) for d in range(3)]
+        b = [min(hi[d], o[d] + ps[d]) for d in range(3)]
+        so = tuple(slice(a[d] - lo[d], b[d] - lo[d]) for d in range(3))
+        th = np.float64(1.0) / np.float64(2 + ranks[L])
+        w[so] = (w[so].astype(np.float64) + th).astype(np.float32)
+    return w
+
+
 def oracle_window(vol, ps, po, method, params, sel, grid, lo, hi, pool=None):
     """Oracle output / weights on the window and the ranks of its patches."""
     idx = covering(sel, grid, ps, po, lo, hi)

@@ -42,6 +42,9 @@ def test_pool_matches_sequential_oracle():
                                            atol=1e-3)
                 np.testing.assert_allclose(w, ref_w[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]],
                                            rtol=1e-6)
+                # the reference's float32 weight sums, bit for bit
+                w32 = OP.window_weights32(ps, po, grid, wr, lo, hi, T)
+                assert np.array_equal(w32, ref_w[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]])
     finally:
         pool.shutdown()
 
