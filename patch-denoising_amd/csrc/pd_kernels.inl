@@ -1415,16 +1415,39 @@ __global__ __launch_bounds__(NT) void tridiag_big_kernel(GramArgs a, int k0) {
       DEp[2 * n + k] = e * e;
       TAUp[k] = t;
     };
-    // ---------------- global phase, steps k < k0
+    // ---------------- global phase, steps k < k0: the rank-2 update of step k
+    // is applied in step k + 1's pass over the trailing matrix, which also
+    // accumulates that step's p (one read and one write of the matrix per
+    // step); row k + 1 gets the update first, as the source of reflector k + 1
+    E* vn = Bp + 2 * n;  // reflector of the current step (vg / wg: the pending update's)
+    bool pending = false;
     for (int k = 0; k < k0 && k < n - 1; ++k) {
       E* rowk = A + (size_t)k * n;  // conj(row k) = column k (the matrix is kept Hermitian)
+      if (pending) {
+        const E vk = vg[k], wk = wg[k];
+        for (int c = k + tid; c < n; c += NT) rowk[c] = rank2_update(rowk[c], vk, conj_(wg[c]), wk, conj_(vg[c]));
+        __syncthreads();
+      }
       const E alpha = conj_(rowk[k + 1]);
       const R dk = re_(rowk[k]);
       R part = R(0);
       for (int c = k + 2 + tid; c < n; c += NT) part += abs2_(rowk[c]);
       const R xn2 = block_sum<NT>(part, redr);  // its barriers order the reads above before the stores below
       const R ar = re_(alpha), ai = im_(alpha);
-      if (!((xn2 > R(0)) || (ai != R(0)))) {  // H_k = I
+      if (!((xn2 > R(0)) || (ai != R(0)))) {  // H_k = I: the pending update alone
+        if (pending) {
+#pragma unroll
+          for (int jj = 0; jj < (kNBig + NT - 1) / NT; ++jj) {
+            const int c = k + 1 + tid + jj * NT;
+            if (c < n) {
+              E* col = A + c;
+              const E cw = conj_(wg[c]), cv = conj_(vg[c]);
+              for (int r = k + 1; r < n; ++r)
+                col[(size_t)r * n] = rank2_update(col[(size_t)r * n], vg[r], cw, wg[r], cv);
+            }
+          }
+          pending = false;
+        }
         for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = zero;
         if (tid == 0) put_step(k, dk, ar, zero);
         __syncthreads();
@@ -1435,67 +1458,88 @@ __global__ __launch_bounds__(NT) void tridiag_big_kernel(GramArgs a, int k0) {
       const E tk = mk<E>((beta - ar) * ibeta, -ai * ibeta);
       const E scal = recip_(alpha - E(beta));
       for (int c = tid; c < n; c += NT)
-        vg[c] = c > k + 1 ? conj_(rowk[c]) * scal : (c == k + 1 ? E(R(1)) : zero);
+        vn[c] = c > k + 1 ? conj_(rowk[c]) * scal : (c == k + 1 ? E(R(1)) : zero);
       __syncthreads();
-      for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = vg[c];
+      for (int c = k + 1 + tid; c < n; c += NT) rowk[c] = vn[c];
       if (tid == 0) put_step(k, dk, beta, tk);
+      // the pending update and p = tau A v in one pass over the trailing columns
       E pc[(kNBig + NT - 1) / NT];
 #pragma unroll
-      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
-        const int c = k + 1 + tid + j * NT;
+      for (int jj = 0; jj < (kNBig + NT - 1) / NT; ++jj) {
+        const int c = k + 1 + tid + jj * NT;
         E sacc = zero;
         if (c < n) {
-          const E* col = A + c;
-          int r = k + 1;
-          for (; r + 4 <= n; r += 4) {
-            E x[4];
+          E* col = A + c;
+          if (pending) {
+            const E cw = conj_(wg[c]), cv = conj_(vg[c]);
+            int r = k + 1;
+            for (; r + 4 <= n; r += 4) {
+              E x[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
+              for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) sacc += conj_(x[u]) * vg[r + u];
+              for (int u = 0; u < 4; ++u) {
+                x[u] = rank2_update(x[u], vg[r + u], cw, wg[r + u], cv);
+                col[(size_t)(r + u) * n] = x[u];
+                sacc += conj_(x[u]) * vn[r + u];
+              }
+            }
+            for (; r < n; ++r) {
+              const E x = rank2_update(col[(size_t)r * n], vg[r], cw, wg[r], cv);
+              col[(size_t)r * n] = x;
+              sacc += conj_(x) * vn[r];
+            }
+          } else {
+            int r = k + 1;
+            for (; r + 4 <= n; r += 4) {
+              E x[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) sacc += conj_(x[u]) * vn[r + u];
+            }
+            for (; r < n; ++r) sacc += conj_(col[(size_t)r * n]) * vn[r];
           }
-          for (; r < n; ++r) sacc += conj_(col[(size_t)r * n]) * vg[r];
         }
-        pc[j] = tk * sacc;
+        pc[jj] = tk * sacc;
       }
       E pv = zero;
 #pragma unroll
-      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
-        const int c = k + 1 + tid + j * NT;
-        if (c < n) pv += conj_(pc[j]) * vg[c];
+      for (int jj = 0; jj < (kNBig + NT - 1) / NT; ++jj) {
+        const int c = k + 1 + tid + jj * NT;
+        if (c < n) pv += conj_(pc[jj]) * vn[c];
       }
-      pv = block_sum<NT>(pv, rede);
+      pv = block_sum<NT>(pv, rede);  // every thread is past its reads of vg / wg
       const E kc = (R(-0.5) * tk) * pv;
+      for (int c = tid; c < n; c += NT) vg[c] = vn[c];
 #pragma unroll
-      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
-        const int c = k + 1 + tid + j * NT;
-        if (c < n) wg[c] = pc[j] + kc * vg[c];
+      for (int jj = 0; jj < (kNBig + NT - 1) / NT; ++jj) {
+        const int c = k + 1 + tid + jj * NT;
+        if (c < n) wg[c] = pc[jj] + kc * vn[c];
       }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < (kNBig + NT - 1) / NT; ++j) {
-        const int c = k + 1 + tid + j * NT;
-        if (c < n) {
-          E* col = A + c;
-          const E cw = conj_(wg[c]), cv = conj_(vg[c]);
-          int r = k + 1;
-          for (; r + 4 <= n; r += 4) {
-            E x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = col[(size_t)(r + u) * n];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              col[(size_t)(r + u) * n] = rank2_update(x[u], vg[r + u], cw, wg[r + u], cv);
-          }
-          for (; r < n; ++r) col[(size_t)r * n] = rank2_update(col[(size_t)r * n], vg[r], cw, wg[r], cv);
-        }
-      }
+      for (int c = tid; c <= k && c < n; c += NT) wg[c] = zero;
+      pending = true;
       __syncthreads();
     }
     // ---------------- LDS phase: load the lower triangle of A[k0:, k0:]
+    // (with the last global step's update, its v / w first copied out of the
+    // area the triangle overwrites)
+    if (pending) {
+      for (int i = tid; i < m0; i += NT) {
+        v[i] = vg[k0 + i];
+        w[i] = wg[k0 + i];
+      }
+      __syncthreads();
+    }
     for (int i = 0; i < m0; ++i) {
       const E* row = A + (size_t)(k0 + i) * n + k0;
-      for (int j = tid; j <= i; j += NT) Bp[pk_col(j, m0) + (i - j)] = row[j];
+      if (pending) {
+        const E vi = v[i], wi = w[i];
+        for (int j = tid; j <= i; j += NT)
+          Bp[pk_col(j, m0) + (i - j)] = rank2_update(row[j], vi, conj_(w[j]), wi, conj_(v[j]));
+      } else {
+        for (int j = tid; j <= i; j += NT) Bp[pk_col(j, m0) + (i - j)] = row[j];
+      }
     }
     __syncthreads();
     for (int kk = 0; kk < m0 - 1; ++kk) {
@@ -3279,10 +3323,13 @@ int launch_tridiag_big(const Plan& pl, GramArgs ga, hipStream_t st) {
   // costs at most a few more global-memory steps
   int per_cu = 1, k0 = tridiag_big_k0(pl.n, sizeof(E));
   const int k0b = tridiag_big_k0(pl.n, sizeof(E), 2);
-  if (k0b <= k0 + std::max(2, pl.n / 32)) {
+  // the global phase keeps three length-n vectors in the packed triangle's area
+  auto room = [&](int kk) { const size_t m = (size_t)(pl.n - kk); return kk == 0 || m * (m + 1) / 2 >= 3 * (size_t)pl.n; };
+  if (k0b <= k0 + std::max(2, pl.n / 32) && room(k0b)) {
     per_cu = 2;
     k0 = k0b;
   }
+  if (!room(k0)) return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS too small for n = %d", pl.n);
   const size_t lds = tridiag_big_lds_bytes<E>(pl.n, k0);
   if (k0 > pl.n - 2 || (lds + 256) * per_cu > kTBLds)
     return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes (k0 %d)", lds, k0);
