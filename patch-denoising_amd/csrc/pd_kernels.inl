@@ -3124,12 +3124,18 @@ __global__ __launch_bounds__(kNT) void synth_kernel(const IN* __restrict__ vol, 
           }
           for (int c = tid; c < kg; c += kNT) wcs[c] = (R)WCp[k0 + c];
           lds_barrier();
-          // C[r][c] = w_c * sum_t Xc[r][t] Y[t][c]
-          for (int q = tid; q < rc * kg; q += kNT) {
+          // C[r][c] = w_c * sum_t Xc[r][t] Y[t][c]; with few (r, c) pairs each sum
+          // is split over 2 or 4 adjacent lanes (frames t = h mod split),
+          // combined by a fixed butterfly
+          const int npair = rc * kg;
+          const int split = npair * 4 <= kNT ? 4 : (npair * 2 <= kNT ? 2 : 1);
+          for (int q0 = tid; q0 < npair * split; q0 += kNT) {
+            const int q = q0 / split, h = q0 % split;
             const int r = q / kg, c = q % kg;
             E b = E(R(0));
-            for (int t = 0; t < T; ++t) b += Sc[r * ldw + t] * Ys[t * KG + c];
-            B[r * KG + c] = wcs[c] * b;
+            for (int t = h; t < T; t += split) b += Sc[r * ldw + t] * Ys[t * KG + c];
+            for (int o = 1; o < split; o <<= 1) b += shfl_xor_e(b, o);
+            if (h == 0) B[r * KG + c] = wcs[c] * b;
           }
           lds_barrier();
 #pragma unroll
