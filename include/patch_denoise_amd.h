@@ -163,9 +163,11 @@ typedef struct pdamd_problem {
      PDAMD_RECOMBINE_GATHER  every output element sums its covering patches
                              in increasing patch index, no atomics, results
                              repeat bit for bit.  Eligible: float32 data, one
-                             temporal patch, more frames than patch voxels,
-                             weighted / average recombination, MP-PCA or
-                             Hybrid; otherwise PDAMD_EUNSUPPORTED.          */
+                             temporal patch, weighted / average
+                             recombination, MP-PCA or Hybrid, a Gram of at
+                             most 128 (voxel side) or 256 (frame side, at
+                             most 512 frames), an uncapped workspace;
+                             otherwise PDAMD_EUNSUPPORTED.                  */
   int recombine_mode;
 
   /* FP64 re-check of near-threshold decisions (no counterpart in the
