@@ -239,7 +239,9 @@ constexpr int kPF = 3;           // visits whose global loads are in flight ahea
 constexpr int kURow = 12;        // LDS row of one eigenvector row (10 used, 16-B aligned)
 
 __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
-  __shared__ float acc[kSlice * kSlice * kWave];
+  // per lane (frame) the 64 voxels of the slice, in registers (compile-time
+  // indices; each visit adds to the voxels of its box, a wavefront-uniform test)
+  float acc[kSlice * kSlice];
   // visit list (+ padding: empty visits up to a multiple of kPF + 1 and the
   // prefetch distance, so the visit loop has no bounds branches)
   __shared__ int vslot[kVisitCap + 2 * (kPF + 1)];  // slot of each visit
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
   const bool tv = t < T;
   const int tt = tv ? t : 0;
 #pragma unroll
-  for (int v = 0; v < kSlice * kSlice; ++v) acc[v * kWave + lane] = 0.f;
+  for (int v = 0; v < kSlice * kSlice; ++v) acc[v] = 0.f;
   int gxl, gxh, gyl, gyh, gzl, gzh, lo, hi;
   covering(x0, g.p0, g.s0, g.g0, gxl, hi);
   covering(xe - 1, g.p0, g.s0, g.g0, lo, gxh);
@@ -382,22 +384,12 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
         val = fmaf(u2.y, rf[10], val);
         return val;
       };
-      // rows of the box (x), runs of voxels along y four at a time; the LDS
-      // eigenvector row of box voxel (x, y) is (x - xa) * by + (y - ya)
-      for (int x = xa, jr = 0; x < xb; ++x, jr += by) {
-        float* crow = acc + ((x - x0) * kSlice - y0) * kWave + lane;
-        int y = ya;
-        for (; y + 4 <= yb; y += 4) {
-          const int j = jr + (y - ya);
-          float* c0 = crow + y * kWave;
-          const float a0 = c0[0], a1 = c0[kWave], a2 = c0[2 * kWave], a3 = c0[3 * kWave];
-          const float v0 = contrib(j), v1 = contrib(j + 1), v2 = contrib(j + 2), v3 = contrib(j + 3);
-          c0[0] = a0 + v0;
-          c0[kWave] = a1 + v1;
-          c0[2 * kWave] = a2 + v2;
-          c0[3 * kWave] = a3 + v3;
-        }
-        for (; y < yb; ++y) crow[y * kWave] += contrib(jr + (y - ya));
+      // the box's voxels (x, y), each with the LDS eigenvector row
+      // (x - xa) * by + (y - ya); voxels outside the box are skipped
+#pragma unroll
+      for (int v = 0; v < kSlice * kSlice; ++v) {
+        const int x = x0 + v / kSlice, y = y0 + v % kSlice;
+        if (x >= xa && x < xb && y >= ya && y < yb) acc[v] += contrib((x - xa) * by + (y - ya));
       }
       wave_sync();
     };
@@ -411,12 +403,13 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
     wave_sync();
   }
   if (!tv) return;
+#pragma unroll
   for (int v = 0; v < kSlice * kSlice; ++v) {
     const int x = x0 + v / kSlice, y = y0 + v % kSlice;
     if (x >= X || y >= Y) continue;
     const int64_t vox = ((int64_t)x * Y + y) * Z + z;
     const float wv = a.W3[vox];
-    float o = acc[v * kWave + lane] / wv;
+    float o = acc[v] / wv;
     if (a.mask_kind == PDAMD_MASK_SPATIAL && !a.mask[vox]) o = 0.f;
     if (a.mask_kind == PDAMD_MASK_FULL && !a.mask[vox * T + t]) o = 0.f;
     a.out[vox * T + t] = o;
