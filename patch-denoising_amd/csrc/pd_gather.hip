@@ -235,7 +235,7 @@ __global__ void gather_weight_kernel(GatherArgs a) {
 //   3. each box voxel's contribution (11 FMAs per lane, the eigenvector row
 //      read as a broadcast) is added to its LDS accumulator.
 constexpr int kVisitCap = 768;   // visit-list entries per segment of covering patches
-constexpr int kPF = 3;           // visits whose global loads are in flight ahead of the current one
+constexpr int kPF = 1;           // visits whose global loads are in flight ahead of the current one
 constexpr int kURow = 12;        // LDS row of one eigenvector row (10 used, 16-B aligned)
 
 __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
@@ -393,12 +393,12 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
       }
       wave_sync();
     };
-    static_assert(kPF == 3, "the visit loop below is unrolled for kPF + 1 == 4 slots");
+    static_assert(kPF >= 1 && kPF <= 3, "the visit loop below is unrolled for kPF + 1 <= 4 slots");
     for (int v = 0; v < nvp; v += kPF + 1) {
       visit_one(v, std::integral_constant<int, 0>());
       visit_one(v + 1, std::integral_constant<int, 1>());
-      visit_one(v + 2, std::integral_constant<int, 2>());
-      visit_one(v + 3, std::integral_constant<int, 3>());
+      if constexpr (kPF >= 2) visit_one(v + 2, std::integral_constant<int, 2>());
+      if constexpr (kPF >= 3) visit_one(v + 3, std::integral_constant<int, 3>());
     }
     wave_sync();
   }
