@@ -233,7 +233,9 @@ __global__ void gather_weight_kernel(GatherArgs a) {
 //      box voxels) into LDS, every lane its frame's 11 factor rows -- for the
 //      next visit while the current one is accumulated;
 //   3. each box voxel's contribution (11 FMAs per lane, the eigenvector row
-//      read as a broadcast) is added to its LDS accumulator.
+//      read as a broadcast) is added to its accumulator, one register per
+//      slice voxel (the LDS then holds only the visit list and one visit's
+//      eigenvector rows: four wavefronts per SIMD).
 constexpr int kVisitCap = 768;   // visit-list entries per segment of covering patches
 constexpr int kPF = 1;           // visits whose global loads are in flight ahead of the current one
 constexpr int kURow = 12;        // LDS row of one eigenvector row (10 used, 16-B aligned)
