@@ -3251,10 +3251,42 @@ int zero_async(void* p, size_t bytes, hipStream_t st) {
   return PDAMD_OK;
 }
 
-// Workgroups per CU: the caller's override (autotuner) when set, capped by LDS.
-inline int pick_per_cu(int override_, int dflt, size_t lds) {
+// Workgroups of `kern` (`threads` each, `lds` dynamic LDS bytes) that one CU
+// holds at once -- registers, LDS and wave slots together -- or 0 if the
+// runtime cannot say.  Cached per kernel and shape.
+inline int resident_per_cu(const void* kern, int threads, size_t lds) {
+  struct Entry {
+    const void* k;
+    int t;
+    size_t l;
+    int v;
+  };
+  thread_local Entry cache[32];
+  thread_local int used = 0;
+  for (int i = 0; i < used; ++i)
+    if (cache[i].k == kern && cache[i].t == threads && cache[i].l == lds) return cache[i].v;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    nb = 0;
+  }
+  if (used < 32) cache[used++] = {kern, threads, lds, std::max(nb, 0)};
+  return std::max(nb, 0);
+}
+
+// Workgroups per CU of a persistent launch (patches strided over the grid):
+// the caller's override (autotuner) when set, else the default capped by what
+// stays resident -- workgroups beyond that would start only when the first
+// ones have finished their whole share, a second pass at a fraction of the
+// occupancy.  Either way at most what the LDS holds.
+inline int pick_per_cu(int override_, int dflt, size_t lds, const void* kern, int threads) {
   const int cap = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
-  return std::max(1, std::min(override_ > 0 ? override_ : dflt, cap));
+  int v = override_ > 0 ? override_ : dflt;
+  if (override_ <= 0) {
+    const int res = resident_per_cu(kern, threads, lds);
+    if (res > 0) v = std::min(v, res);
+  }
+  return std::max(1, std::min(v, cap));
 }
 
 int grid_for(int64_t np, int per_cu) {
@@ -3274,7 +3306,7 @@ int launch_gram(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st) {
   if (lds > 160 * 1024) return fail(PDAMD_EUNSUPPORTED, "gram LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = gram_kernel<IN, E, TSIDE, TILE, BF>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = pick_per_cu(pl.per_cu[0], 8, lds);
+  const int per_cu = pick_per_cu(pl.per_cu[0], 8, lds, (const void*)kern, kNT);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -3287,7 +3319,7 @@ int launch_tridiag(const Plan& pl, GramArgs ga, hipStream_t st) {
     return fail(PDAMD_EUNSUPPORTED, "tridiagonalisation LDS %zu bytes exceeds 160 KiB", lds);
   auto kern = tridiag_kernel<E, RJ>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = pick_per_cu(pl.per_cu[1], 8, lds);
+  const int per_cu = pick_per_cu(pl.per_cu[1], 8, lds, (const void*)kern, kNT);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -3308,7 +3340,7 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
       if (lds1 <= 160 * 1024) {
         auto k1 = bf ? gram_big1_kernel<IN, E, TSIDE, true> : gram_big1_kernel<IN, E, TSIDE, false>;
         HIP_TRY(hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
-        const int per_cu = pick_per_cu(pl.per_cu[0], 2, lds1);
+        const int per_cu = pick_per_cu(pl.per_cu[0], 2, lds1, (const void*)k1, kG1T);
         k1<<<grid_for(ga.np, per_cu), kG1T, lds1, st>>>(reinterpret_cast<const IN*>(vol), ga);
         HIP_TRY(hipGetLastError());
         return PDAMD_OK;
@@ -3317,7 +3349,7 @@ int launch_gram_big(const Plan& pl, const void* vol, GramArgs ga, hipStream_t st
   }
   auto kern = bf ? gram_big_kernel<IN, E, TSIDE, F32> : gram_big_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = pick_per_cu(pl.per_cu[0], 4, lds);
+  const int per_cu = pick_per_cu(pl.per_cu[0], 4, lds, (const void*)kern, kNT);
   kern<<<grid_for(ga.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), ga);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -3422,7 +3454,7 @@ int launch_eig(const Plan& pl, EigArgs ea, hipStream_t st) {
                   : (ad ? trieig_kernel<E, RT, kNBig / kWave, 1, true>
                         : trieig_kernel<E, RT, kNBig / kWave, 1>);
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = pick_per_cu(pl.per_cu[2], 16, lds);
+  const int per_cu = pick_per_cu(pl.per_cu[2], 16, lds, (const void*)kern, kWave);
   kern<<<grid_for(ea.np, per_cu), kWave, lds, st>>>(ea);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
@@ -3469,7 +3501,7 @@ int launch_synth(const Plan& pl, const void* vol, SynthArgs sa, hipStream_t st) 
               : huge ? synth_kernel<IN, E, TSIDE, kSynSCHuge, kSynKGBig>
               : big ? synth_kernel<IN, E, TSIDE, kSynSCBig, kSynKGBig> : synth_kernel<IN, E, TSIDE>;
   HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = pick_per_cu(pl.per_cu[3], 8, lds);
+  const int per_cu = pick_per_cu(pl.per_cu[3], 8, lds, (const void*)kern, kNT);
   kern<<<grid_for(sa.np, per_cu), kNT, lds, st>>>(reinterpret_cast<const IN*>(vol), sa);
   HIP_TRY(hipGetLastError());
   return PDAMD_OK;
