@@ -1713,11 +1713,16 @@ template <> struct Lim<double> {
   __device__ static double tiny() { return DBL_MIN; }
 };
 
-__device__ __forceinline__ float fdiv_(float a, float b) { return __fdividef(a, b); }
+// FP32 a / b in the recurrences (Sturm counts, LU and twisted factorizations
+// of the tridiagonal): v_rcp_f32 (1 ulp) times a, two instructions where the
+// correctly rounded quotient takes ten (div_scale x 2, rcp, 4 FMAs, div_fmas,
+// div_fixup).  The recurrences are backward stable for quotients of a few ulp
+// relative error; b is kept away from 0 by the callers' pivmin / guards.
+__device__ __forceinline__ float fdiv_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double fdiv_(double a, double b) { return a / b; }
-// a / b from the hardware reciprocal refined by one Newton step (a few ulp,
-// no special-case handling; b is kept away from 0 by the callers' pivmin).
-__device__ __forceinline__ float qdiv_(float a, float b) { return __fdividef(a, b); }
+// a / b from the hardware reciprocal (FP64: refined by one Newton step; a few
+// ulp, no special-case handling; b is kept away from 0 by the callers' pivmin).
+__device__ __forceinline__ float qdiv_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double qdiv_(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
   r = fma(fma(-b, r, 1.0), r, r);
@@ -3251,6 +3256,15 @@ int zero_async(void* p, size_t bytes, hipStream_t st) {
   return PDAMD_OK;
 }
 
+// LDS a workgroup occupies: static + dynamic bytes in 1280-byte granules.
+// (An eigen launch of 16,240 bytes per workgroup measured as if nine fit in
+// the 160 KiB of a CU, not ten: 10 x 13 granules exceed it.)
+constexpr size_t kLdsGranule = 1280;
+inline int lds_per_cu(size_t bytes) {
+  const size_t g = (bytes + kLdsGranule - 1) / kLdsGranule * kLdsGranule;
+  return g ? (int)((160 * 1024) / g) : 1 << 20;
+}
+
 // Workgroups of `kern` (`threads` each, `lds` dynamic LDS bytes) that one CU
 // holds at once -- registers, LDS and wave slots together -- or 0 if the
 // runtime cannot say.  Cached per kernel and shape.
@@ -3266,9 +3280,13 @@ inline int resident_per_cu(const void* kern, int threads, size_t lds) {
   for (int i = 0; i < used; ++i)
     if (cache[i].k == kern && cache[i].t == threads && cache[i].l == lds) return cache[i].v;
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds) != hipSuccess) {
+  hipFuncAttributes fa{};
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds) != hipSuccess ||
+      hipFuncGetAttributes(&fa, kern) != hipSuccess) {
     (void)hipGetLastError();
     nb = 0;
+  } else {
+    nb = std::min(nb, lds_per_cu(lds + fa.sharedSizeBytes));
   }
   if (used < 32) cache[used++] = {kern, threads, lds, std::max(nb, 0)};
   return std::max(nb, 0);
