@@ -240,7 +240,7 @@ constexpr int kVisitCap = 768;   // visit-list entries per segment of covering p
 constexpr int kPF = 1;           // visits whose global loads are in flight ahead of the current one
 constexpr int kURow = 12;        // LDS row of one eigenvector row (10 used, 16-B aligned)
 
-__global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
+__global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a) {
   // per lane (frame) the 64 voxels of the slice, in registers (compile-time
   // indices; each visit adds to the voxels of its box, a wavefront-uniform test)
   float acc[kSlice * kSlice];
@@ -248,7 +248,10 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
   // prefetch distance, so the visit loop has no bounds branches)
   __shared__ int vslot[kVisitCap + 2 * (kPF + 1)];  // slot of each visit
   __shared__ int vcode[kVisitCap + 2 * (kPF + 1)];  // (gx-gxl) << 20 | (gy-gyl) << 10 | (gz-gzl), -1: empty
-  __shared__ __align__(16) float urow[kSlice * kSlice * kURow];
+  // eigenvector rows of the current visit's box voxels, after kSlice rows of
+  // padding (a box row's base may lie up to kSlice - 1 rows before its first
+  // voxel's row)
+  __shared__ __align__(16) float urow[(kSlice + kSlice * kSlice) * kURow];
   const Geom& g = a.g;
   const int lane = threadIdx.x;
   const int X = (int)a.shape[0], Y = (int)a.shape[1], Z = (int)a.shape[2];
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
       constexpr int b = decltype(B)::value;
       constexpr int bn = (b + kPF) % (kPF + 1);
       {  // eigenvector rows of this visit -> LDS (lane j: box voxel j)
-        float* dst = urow + lane * kURow;
+        float* dst = urow + (kSlice + lane) * kURow;
 #pragma unroll
         for (int k = 0; k < KM; ++k) dst[k] = uq[b][k];
       }
@@ -370,8 +373,8 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
       const int by = yb - ya;
       // box voxels four at a time: accumulator and eigenvector-row reads
       // first (distinct voxels, so the reads need not wait on the writes)
-      auto contrib = [&](int j) {
-        const float4* us = reinterpret_cast<const float4*>(urow + j * kURow);
+      auto contrib = [&](const float* ur) {
+        const float4* us = reinterpret_cast<const float4*>(ur);
         const float4 u0 = us[0], u1 = us[1], u2 = us[2];
         float val = rf[0];
         val = fmaf(u0.x, rf[1], val);
@@ -387,11 +390,22 @@ __global__ __launch_bounds__(kWave) void gather_recombine_kernel(GatherArgs a) {
         return val;
       };
       // the box's voxels (x, y), each with the LDS eigenvector row
-      // (x - xa) * by + (y - ya); voxels outside the box are skipped
+      // (x - xa) * by + (y - ya): rows and columns of the tile inside the box
+      // as 8-bit masks (one scalar bit test per row and per voxel), a row's
+      // eigenvector rows from one base plus the column's constant offset
+      auto span = [](int lo, int hi) {  // bits [lo, hi) of 0 .. kSlice
+        lo = min(max(lo, 0), kSlice);
+        hi = min(max(hi, 0), kSlice);
+        return ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+      };
+      const unsigned rm = span(xa - x0, xb - x0), cm = span(ya - y0, yb - y0);
 #pragma unroll
-      for (int v = 0; v < kSlice * kSlice; ++v) {
-        const int x = x0 + v / kSlice, y = y0 + v % kSlice;
-        if (x >= xa && x < xb && y >= ya && y < yb) acc[v] += contrib((x - xa) * by + (y - ya));
+      for (int rx = 0; rx < kSlice; ++rx) {
+        if (!(rm & (1u << rx))) continue;
+        const float* ur = urow + (kSlice + (x0 + rx - xa) * by - (ya - y0)) * kURow;
+#pragma unroll
+        for (int ry = 0; ry < kSlice; ++ry)
+          if (cm & (1u << ry)) acc[rx * kSlice + ry] += contrib(ur + ry * kURow);
       }
       wave_sync();
     };
