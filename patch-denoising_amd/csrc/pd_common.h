@@ -200,4 +200,15 @@ __device__ __forceinline__ int64_t row_offset(const Geom& g, int r) {
   return (int64_t)i0 * g.vs0 + (int64_t)i1 * g.vs1 + (int64_t)i2 * g.vs2;
 }
 
+// This workgroup's rank when the grid is numbered XCD by XCD.  Workgroups are
+// dispatched round-robin over the 8 XCDs, each with its own L2; with patch
+// p = rank (+ a multiple of the grid), the workgroups of one XCD work on
+// consecutive patches at any time -- neighbours that share most of their
+// voxels, so the volume reads hit that XCD's L2 instead of going past it.
+__device__ __forceinline__ int xcd_rank() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int per = G / 8, rem = G % 8, x = b % 8;
+  return x * per + (x < rem ? x : rem) + b / 8;
+}
+
 }  // namespace pd

@@ -37,7 +37,7 @@ __device__ __forceinline__ void covering(int c, int p, int s, int gcount, int& l
 // independent data loads are in flight per row).
 template <int NCH>
 __global__ __launch_bounds__(kWave) void gather_factor_kernel(GatherArgs a) {
-  const int p = blockIdx.x;  // chunk-local patch, or list entry
+  const int p = xcd_rank();  // chunk-local patch, or list entry (XCD-major order)
   if (a.list && p >= *a.list_cnt) return;
   const int slot = a.list ? a.list[p] : a.slot0 + p;
   const int lane = threadIdx.x;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kWave) void gather_factor_kernel(GatherArgs a) {
 // its row's frames (four per load when rows are 16-byte aligned).
 __global__ __launch_bounds__(kWave) void gather_factor_t_kernel(GatherArgs a) {
   extern __shared__ __align__(16) float fts[];
-  const int p = blockIdx.x;  // chunk-local patch, or list entry
+  const int p = xcd_rank();  // chunk-local patch, or list entry (XCD-major order)
   if (a.list && p >= *a.list_cnt) return;
   const int slot = a.list ? a.list[p] : a.slot0 + p;
   const int lane = threadIdx.x;

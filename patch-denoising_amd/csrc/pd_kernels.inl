@@ -488,7 +488,7 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
   __bf16* Sh = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;  // Sh[i][k]
   __bf16* Sl = BFM ? lds_carve<__bf16>(cur, (size_t)NB * kBfW) : nullptr;
 
-  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
+  for (int p = xcd_rank(), np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
@@ -886,7 +886,7 @@ __global__ __launch_bounds__(kNT) void gram_big_kernel(const IN* __restrict__ vo
   };
   const int nb = (n + kGBT - 1) / kGBT;
   const int ntile = nb * (nb + 1) / 2;
-  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
+  for (int p = xcd_rank(), np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
@@ -1210,7 +1210,7 @@ __global__ __launch_bounds__(kG1T) void gram_big1_kernel(const IN* __restrict__ 
     bi[b] = r;
     bj[b] = blk - r * (r + 1) / 2;
   }
-  for (int p = blockIdx.x, np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
+  for (int p = xcd_rank(), np_ = a.np_dev ? *a.np_dev : a.np; p < np_; p += gridDim.x) {
     int64_t sp;
     int t0;
     patch_origin(g, a.idx[p], sp, t0);
