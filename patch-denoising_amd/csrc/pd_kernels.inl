@@ -699,8 +699,12 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
     //      reference's eigen path)
     E* G = reinterpret_cast<E*>(a.H) + (size_t)p * n * n;
     if constexpr (M16) {
-      // 16x16 MFMA output: accumulator q of lane l is C[4(l/16) + q][l%16];
-      // off-diagonal tiles are written to both triangles
+      // 16x16 MFMA output: accumulator q of lane l is C[4(l/16) + q][l%16].
+      // Only the lower-triangle tiles are written (diagonal tiles whole): the
+      // one-wavefront Householder that takes every FP32 Gram of this size
+      // reads the blocks on and below the diagonal only, and the mirrored
+      // stores (4-byte writes a row apart) cost more than the rest of the
+      // Gram's output
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
         int I, J;
@@ -718,7 +722,6 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
             if (a.scale) v = div_n(v, M - 1);
             float* Gf = reinterpret_cast<float*>(G);
             Gf[(size_t)r * n + c] = v;
-            if (I != J) Gf[(size_t)c * n + r] = v;
           }
         }
       }
