@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
   float acc[kSlice * kSlice];
   // visit list (+ padding: empty visits up to a multiple of kPF + 1 and the
   // prefetch distance, so the visit loop has no bounds branches)
-  __shared__ int vslot[kVisitCap + 2 * (kPF + 1)];  // slot of each visit
+  __shared__ int vslot[kVisitCap + 2 * (kPF + 1)];  // slot * 16 + kept components of each visit
   __shared__ int vcode[kVisitCap + 2 * (kPF + 1)];  // (gx-gxl) << 20 | (gy-gyl) << 10 | (gz-gzl), -1: empty
   // eigenvector rows of the current visit's box voxels, after kSlice rows of
   // padding (a box row's base may lie up to kSlice - 1 rows before its first
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
       const unsigned long long sel = __ballot(e >= 0);
       const int pos = nvis + __popcll(sel & ((1ull << lane) - 1ull));
       if (e >= 0) {
-        vslot[pos] = e >> 4;
+        vslot[pos] = e;
         vcode[pos] = code;
       }
       nvis += __popcll(sel);
@@ -329,17 +329,21 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
     if (nvis == 0) continue;
     const int nvp = (nvis + kPF) / (kPF + 1) * (kPF + 1);  // visits processed (padded)
     for (int q = nvis + lane; q < nvp + kPF; q += kWave) {
-      vslot[q] = vslot[0];
+      vslot[q] = vslot[0] & ~15;  // no components: loads only the mean row
       vcode[q] = -1;
     }
     wave_sync();
     // 2./3. visits, the loads of the next kPF ones in flight (rotating registers)
     float rq[kPF + 1][KM + 1], uq[kPF + 1][KM];
+    // factor rows past a patch's kept components are zero: not loaded
     auto fetch = [&](int v, float (&r)[KM + 1], float (&u)[KM]) {
-      const int slot = vslot[v];
+      const int e = vslot[v], slot = e >> 4, nc = e & 15;
       const float* Rs = RF + (size_t)slot * KR + tt;
 #pragma unroll
-      for (int k = 0; k <= KM; ++k) r[k] = Rs[(size_t)k * T];
+      for (int k = 0; k <= KM; ++k) {
+        if (k <= nc) r[k] = Rs[(size_t)k * T];
+        else r[k] = 0.f;
+      }
       int ox, oy, xa, xb, ya, yb, zi;
       box(vcode[v], ox, oy, xa, xb, ya, yb, zi);
       const int by = yb - ya, nbx = (xb - xa) * by;
@@ -348,7 +352,10 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
           nbx > 0 ? ((xa - ox + j / max(by, 1)) * g.p1 + (ya - oy + j % max(by, 1))) * g.p2 + zi : 0;
       const float* Ur = Yb + ((size_t)slot * a.n + row) * KM;
 #pragma unroll
-      for (int k = 0; k < KM; ++k) u[k] = Ur[k];
+      for (int k = 0; k < KM; ++k) {
+        if (k < nc) u[k] = Ur[k];
+        else u[k] = 0.f;
+      }
     };
 #pragma unroll
     for (int d = 0; d < kPF; ++d) fetch(d, rq[d], uq[d]);
@@ -373,20 +380,26 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
       const int by = yb - ya;
       // box voxels four at a time: accumulator and eigenvector-row reads
       // first (distinct voxels, so the reads need not wait on the writes)
+      // a visit keeping at most 4 components skips the zero terms 5 .. 10
+      // (adding exact zeros changes at most the sign of a zero sum)
+      const bool wide = (vslot[v] & 15) > 4;
       auto contrib = [&](const float* ur) {
         const float4* us = reinterpret_cast<const float4*>(ur);
-        const float4 u0 = us[0], u1 = us[1], u2 = us[2];
+        const float4 u0 = us[0];
         float val = rf[0];
         val = fmaf(u0.x, rf[1], val);
         val = fmaf(u0.y, rf[2], val);
         val = fmaf(u0.z, rf[3], val);
         val = fmaf(u0.w, rf[4], val);
-        val = fmaf(u1.x, rf[5], val);
-        val = fmaf(u1.y, rf[6], val);
-        val = fmaf(u1.z, rf[7], val);
-        val = fmaf(u1.w, rf[8], val);
-        val = fmaf(u2.x, rf[9], val);
-        val = fmaf(u2.y, rf[10], val);
+        if (wide) {
+          const float4 u1 = us[1], u2 = us[2];
+          val = fmaf(u1.x, rf[5], val);
+          val = fmaf(u1.y, rf[6], val);
+          val = fmaf(u1.z, rf[7], val);
+          val = fmaf(u1.w, rf[8], val);
+          val = fmaf(u2.x, rf[9], val);
+          val = fmaf(u2.y, rf[10], val);
+        }
         return val;
       };
       // the box's voxels (x, y), each with the LDS eigenvector row
@@ -399,14 +412,17 @@ __global__ __launch_bounds__(kWave, 4) void gather_recombine_kernel(GatherArgs a
         return ((1u << hi) - 1u) & ~((1u << lo) - 1u);
       };
       const unsigned rm = span(xa - x0, xb - x0), cm = span(ya - y0, yb - y0);
+      auto voxels = [&](auto f) {
 #pragma unroll
-      for (int rx = 0; rx < kSlice; ++rx) {
-        if (!(rm & (1u << rx))) continue;
-        const float* ur = urow + (kSlice + (x0 + rx - xa) * by - (ya - y0)) * kURow;
+        for (int rx = 0; rx < kSlice; ++rx) {
+          if (!(rm & (1u << rx))) continue;
+          const float* ur = urow + (kSlice + (x0 + rx - xa) * by - (ya - y0)) * kURow;
 #pragma unroll
-        for (int ry = 0; ry < kSlice; ++ry)
-          if (cm & (1u << ry)) acc[rx * kSlice + ry] += contrib(ur + ry * kURow);
-      }
+          for (int ry = 0; ry < kSlice; ++ry)
+            if (cm & (1u << ry)) acc[rx * kSlice + ry] += f(ur + ry * kURow);
+        }
+      };
+      voxels(contrib);
       wave_sync();
     };
     static_assert(kPF >= 1 && kPF <= 3, "the visit loop below is unrolled for kPF + 1 <= 4 slots");
