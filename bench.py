@@ -533,8 +533,8 @@ def main():
                     help="also run one plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is described in DESIGN.md, not built).", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: patches would "
+              "shard across GPUs by volume region, which is not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
