@@ -2769,6 +2769,95 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
             if (c < kb && row_of(sl) < n) Zp[(size_t)row_of(sl) * a.kmax + c0 + c] = z[sl][c];
         __syncthreads();
       }
+      if constexpr (std::is_same<E, float>::value && NR == 2 && KBC % 2 == 0) {
+        // FP32, n <= 128: components in pairs, one per half wavefront -- lane
+        // 32 h + i holds rows i + 32 s (s < 4) of components 2 j + h -- so an
+        // instruction serves two components and each dot product is a sum
+        // over 32 lanes (the 64-lane layout above: 6 levels, one component).
+        // z moves into that layout through LDS (zb).
+        constexpr int KH = KBC / 2, PF2 = 4;
+        const int hf = lane >> 5, il = lane & 31;
+        wave_sync();
+#pragma unroll
+        for (int sl = 0; sl < NR; ++sl)
+#pragma unroll
+          for (int c = 0; c < KBC; ++c)
+            if (c < kb && row_of(sl) < n) zb[row_of(sl) * KB + c] = z[sl][c];
+        wave_sync();
+        float y2[4][KH];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+          for (int j = 0; j < KH; ++j) {
+            const int r = il + 32 * s4, c = 2 * j + hf;
+            y2[s4][j] = (c < kb && r < n) ? zb[r * KB + c] : 0.f;
+          }
+        // reflectors kbase, kbase - 1, ... (packed rows, indexed by column)
+        float vr2[4][PF2], tv2[PF2];
+        auto fetch2 = [&](int kbase) {
+#pragma unroll
+          for (int u = 0; u < PF2; ++u) {
+            const int k = kbase - u;
+            if (k >= 0) {
+              const float* row = reinterpret_cast<const float*>(Hp) + hh_row(k, n) - (k + 1);
+              tv2[u] = reinterpret_cast<const float*>(TAUp)[k];
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) {
+                const int r = il + 32 * s4;
+                vr2[s4][u] = (r > k + 1 && r < n) ? row[r] : (r == k + 1 ? 1.f : 0.f);
+              }
+            } else {
+              tv2[u] = 0.f;
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) vr2[s4][u] = 0.f;
+            }
+          }
+        };
+        // sum over this lane's half wavefront: row sums of 16 lanes (DPP),
+        // then the half's two rows; bit-identical in all lanes of the half
+        auto half_sum = [&](float v) {
+          v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+          v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+          v += dpp_f<0x141>(v);  // row_half_mirror
+          v += dpp_f<0x140>(v);  // row_mirror
+          const float s0 = readlane_(v, 0) + readlane_(v, 16);
+          const float s1 = readlane_(v, 32) + readlane_(v, 48);
+          return hf ? s1 : s0;
+        };
+        fetch2(n - 2);
+        for (int kbase = n - 2; kbase >= 0; kbase -= PF2) {
+          float cv[4][PF2], ct[PF2];
+#pragma unroll
+          for (int u = 0; u < PF2; ++u) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) cv[s4][u] = vr2[s4][u];
+            ct[u] = tv2[u];
+          }
+          if (kbase - PF2 >= 0) fetch2(kbase - PF2);
+#pragma unroll
+          for (int u = 0; u < PF2; ++u) {
+            const float tk = ct[u];
+            if (tk == 0.f) continue;
+#pragma unroll
+            for (int j = 0; j < KH; ++j) {
+              if (2 * j >= kb) break;
+              float t = cv[0][u] * y2[0][j];
+#pragma unroll
+              for (int s4 = 1; s4 < 4; ++s4) t = fmaf(cv[s4][u], y2[s4][j], t);
+              const float sdot = tk * half_sum(t);
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) y2[s4][j] = fmaf(-cv[s4][u], sdot, y2[s4][j]);
+            }
+          }
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+          for (int j = 0; j < KH; ++j) {
+            const int r = il + 32 * s4, c = 2 * j + hf;
+            if (c < kb && r < n) Yp[(size_t)r * a.kmax + c0 + c] = y2[s4][j];
+          }
+      } else {
       // back-transformation y = H_0 ... H_{n-2} z, reflectors prefetched PF ahead
       E y[NR][KBC];
 #pragma unroll
@@ -2852,6 +2941,7 @@ __global__ __launch_bounds__(kWave) void trieig_kernel(EigArgs a) {
 #pragma unroll
         for (int c = 0; c < KBC; ++c)
           if (c < kb && row_of(sl) < n) Yp[(size_t)row_of(sl) * a.kmax + c0 + c] = y[sl][c];
+      }
       wave_sync();
     }
     if (a.zero_fill) {  // factor layout of the gather recombination: clean unused columns
