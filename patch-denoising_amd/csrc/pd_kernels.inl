@@ -452,6 +452,41 @@ __device__ __forceinline__ void tri16_accum(const float* __restrict__ S, int ldw
   }
 }
 
+// M-side FP32 16 x 16 tiles: the staged chunk voxel-major, S[i * kPLD + p8(t)]
+// for Gram index i and chunk frame t, where p8 stores the frames f and f + 4
+// of each 8-frame group side by side: one 64-bit read gives a lane its
+// operands for two consecutive MFMA steps (the same operands in the same
+// order as a read per step).  kPLD = 66: a 16-lane read is bank-conflict free.
+constexpr int kPLD = 66;
+__device__ __forceinline__ int p8(int t) { return (t & ~7) | ((t & 3) << 1) | ((t >> 2) & 1); }
+
+template <int W>
+__device__ __forceinline__ void tri16_accum_p8(const float* __restrict__ S, int kcount, int lane,
+                                               const float* __restrict__ mean,
+                                               floatx4 (&acc)[9]) {
+  const int j = lane & 15, g = lane >> 4;
+  for (int k8 = 0; k8 < kcount; k8 += 8) {
+    const float2 m2 = *reinterpret_cast<const float2*>(mean + k8 + 2 * g);
+    float2 v2[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      v2[b] = tri16_uses<W>(b) ? *reinterpret_cast<const float2*>(S + (16 * b + j) * kPLD + k8 + 2 * g)
+                               : float2{0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool kv = k8 + 4 * h + g < kcount;  // this lane's frame at this step
+      float cur[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        cur[b] = (tri16_uses<W>(b) && kv) ? (h ? v2[b].y - m2.y : v2[b].x - m2.x) : 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[Tri16<W>::I[t]], cur[Tri16<W>::J[t]],
+                                                      acc[t], 0, 0, 0);
+    }
+  }
+}
+
 template <class IN, class E, bool TSIDE, int TILE, bool BF = false>
 __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol,
                                                            GramArgs a) {
@@ -481,7 +516,9 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  IN* Sraw = lds_carve<IN>(cur, (size_t)SC * LDS_W);
+  // M-side FP32 16 x 16 tiles (P8): the voxel-major permuted staging
+  constexpr bool P8 = M16 && !TSIDE && SC == 64;
+  IN* Sraw = lds_carve<IN>(cur, P8 ? (size_t)NB * kPLD : (size_t)SC * LDS_W);
   E* Sc = SAME ? reinterpret_cast<E*>(Sraw) : lds_carve<E>(cur, (size_t)SC * LDS_W);
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? g.M : NB);  // row offsets of the patch
@@ -541,6 +578,15 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
       } else if constexpr (M16) {  // Sk: the raw staging, centred on the fly
         const float* S = reinterpret_cast<const float*>(Sk);
         const float* mf = reinterpret_cast<const float*>(mean);
+        if constexpr (!TSIDE && SC == 64) {
+          switch (wv) {
+            case 0: tri16_accum_p8<0>(S, kcount, lane, mf, m16); break;
+            case 1: tri16_accum_p8<1>(S, kcount, lane, mf, m16); break;
+            case 2: tri16_accum_p8<2>(S, kcount, lane, mf, m16); break;
+            default: tri16_accum_p8<3>(S, kcount, lane, mf, m16); break;
+          }
+          return;
+        }
         switch (wv) {
           case 0: tri16_accum<0, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
           case 1: tri16_accum<1, !TSIDE>(S, LDS_W, kcount, lane, mf, m16); break;
@@ -572,13 +618,22 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
           const int q = tid + u * kNT;
-          if (q < SC * NB) Sraw[(q % SC) * LDS_W + q / SC] = buf[u];
+          if (q < SC * NB) {
+            if constexpr (P8) Sraw[(q / SC) * kPLD + p8(q % SC)] = buf[u];
+            else Sraw[(q % SC) * LDS_W + q / SC] = buf[u];
+          }
         }
         lds_barrier();
         if (c0 + SC < T) load(c0 + SC);
         if (tid < tc) {  // sequential over rows, as numpy's axis-0 reduction
-          const IN mv = div_n(seq_add(Sraw[tid * LDS_W], Sraw + tid * LDS_W + 1, M - 1, 1), M);
-          mean[tid] = mv;
+          IN mv;
+          if constexpr (P8) {
+            mv = div_n(seq_add(Sraw[p8(tid)], Sraw + kPLD + p8(tid), M - 1, kPLD), M);
+            mean[p8(tid)] = mv;
+          } else {
+            mv = div_n(seq_add(Sraw[tid * LDS_W], Sraw + tid * LDS_W + 1, M - 1, 1), M);
+            mean[tid] = mv;
+          }
           if (a.MEANW) reinterpret_cast<IN*>(a.MEANW)[(size_t)p * T + c0 + tid] = mv;
         }
         lds_barrier();
@@ -767,7 +822,9 @@ size_t gram_lds_bytes(int n, int M) {
   constexpr int LDS_W = NB + 1;
   constexpr bool SAME = std::is_same<IN, E>::value;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  size_t s = al((size_t)SC * LDS_W * sizeof(IN)) + (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
+  constexpr bool P8 = std::is_same<E, float>::value && TILE == 8 && !BF && !TSIDE && SC == 64;
+  size_t s = al((P8 ? (size_t)NB * kPLD : (size_t)SC * LDS_W) * sizeof(IN)) +
+             (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? (size_t)M : (size_t)NB) * sizeof(int64_t));
   if (BF && std::is_same<E, float>::value && TILE == 8 && SC == 32)
