@@ -137,3 +137,16 @@ def test_prior_variance_map():
     np.testing.assert_array_equal(spatial_variance_map(m, (4, 5, 6, 10)), (m**2).astype(np.float64))
     with pytest.raises(TypeError):
         spatial_variance_map(1, (4, 5, 6, 10))  # a Python int fails like the reference
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits with status 2 and says so
+    (before any GPU or torch import)."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+    assert r.stdout == ""
