@@ -99,8 +99,9 @@ def _tside_volume(shape, T, seed, rank=4):
     return (sig + 25 * rng.standard_normal(shape + (T,))).astype(np.float32)
 
 
-@pytest.mark.parametrize("shape,T,p", [((11, 12, 10), 200, 7), ((12, 12, 10), 60, 5)],
-                         ids=["p7_T200", "p5_T60"])
+@pytest.mark.parametrize("shape,T,p", [((11, 12, 10), 200, 7), ((12, 12, 10), 100, 5),
+                                       ((12, 12, 10), 60, 5)],
+                         ids=["p7_T200", "p5_T100", "p5_T60"])
 def test_gather_tside_repeats_bitwise_and_matches_scatter(shape, T, p):
     from patch_denoise import MPPCADenoiser
 
@@ -128,6 +129,24 @@ def test_gather_tside_weights_bit_exact_against_oracle():
     nbad = int(np.sum(w != ref_w))
     e = rel_l2(out, ref)
     print(f"[gather] T-side 32x32x16x50 p5: weights differing {nbad}, out rel-L2 {e:.2e}",
+          flush=True)
+    assert nbad == 0 and e <= 1e-5
+
+
+def test_gather_tside_gram100_matches_oracle():
+    """Frame-side Gram of 100 (125 voxels > 100 frames, float32): the 16 x 16
+    MFMA Gram tiles, the one-wavefront Householder, the eigen kernel and the
+    T-side gather against the oracle; weights bit-exact, output <= 1e-5."""
+    from oracle import cpu_ref as O
+    from patch_denoise import MPPCADenoiser
+
+    data = _tside_volume((12, 12, 10), 100, 11)
+    out, w, info = _run(MPPCADenoiser(5, 4, threshold_scale=1.0), data, None, "gather")
+    ref, ref_w, _, _ = O.mp_pca(data, 5, 4)
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    nbad = int(np.sum(w != ref_w))
+    e = rel_l2(out, ref)
+    print(f"[gather] T-side 12x12x10x100 p5: weights differing {nbad}, out rel-L2 {e:.2e}",
           flush=True)
     assert nbad == 0 and e <= 1e-5
 
