@@ -452,33 +452,41 @@ __device__ __forceinline__ void tri16_accum(const float* __restrict__ S, int ldw
   }
 }
 
-// M-side FP32 16 x 16 tiles: the staged chunk voxel-major, S[i * kPLD + p8(t)]
-// for Gram index i and chunk frame t, where p8 stores the frames f and f + 4
-// of each 8-frame group side by side: one 64-bit read gives a lane its
-// operands for two consecutive MFMA steps (the same operands in the same
-// order as a read per step).  kPLD = 66: a 16-lane read is bank-conflict free.
-constexpr int kPLD = 66;
+// FP32 16 x 16 tiles: the staged chunk Gram-index-major, S[i * LD + p8(k)]
+// for Gram index i and chunk reduction index k (M-side: frames, LD = 66;
+// T-side: voxel rows, LD = 34), where p8 stores k and k + 4 of each group of
+// 8 side by side: one 64-bit read gives a lane its operands for two
+// consecutive MFMA steps (the same operands in the same order as a read per
+// step).  With these LD a 16-lane read is bank-conflict free.
+constexpr int kPLD = 66, kPLDT = 34;
 __device__ __forceinline__ int p8(int t) { return (t & ~7) | ((t & 3) << 1) | ((t >> 2) & 1); }
 
-template <int W>
+// MSIDE: mean by reduction index (mean[p8(k)]); else by Gram index (mean[i])
+template <int W, bool MSIDE>
 __device__ __forceinline__ void tri16_accum_p8(const float* __restrict__ S, int kcount, int lane,
                                                const float* __restrict__ mean,
                                                floatx4 (&acc)[9]) {
+  constexpr int LD = MSIDE ? kPLD : kPLDT;
   const int j = lane & 15, g = lane >> 4;
+  float mi[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) mi[b] = (!MSIDE && tri16_uses<W>(b)) ? mean[16 * b + j] : 0.f;
   for (int k8 = 0; k8 < kcount; k8 += 8) {
-    const float2 m2 = *reinterpret_cast<const float2*>(mean + k8 + 2 * g);
+    const float2 m2 = MSIDE ? *reinterpret_cast<const float2*>(mean + k8 + 2 * g) : float2{0.f, 0.f};
     float2 v2[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b)
-      v2[b] = tri16_uses<W>(b) ? *reinterpret_cast<const float2*>(S + (16 * b + j) * kPLD + k8 + 2 * g)
+      v2[b] = tri16_uses<W>(b) ? *reinterpret_cast<const float2*>(S + (16 * b + j) * LD + k8 + 2 * g)
                                : float2{0.f, 0.f};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const bool kv = k8 + 4 * h + g < kcount;  // this lane's frame at this step
+      const bool kv = k8 + 4 * h + g < kcount;  // this lane's reduction index at this step
       float cur[8];
 #pragma unroll
-      for (int b = 0; b < 8; ++b)
-        cur[b] = (tri16_uses<W>(b) && kv) ? (h ? v2[b].y - m2.y : v2[b].x - m2.x) : 0.f;
+      for (int b = 0; b < 8; ++b) {
+        const float mk = MSIDE ? (h ? m2.y : m2.x) : mi[b];
+        cur[b] = (tri16_uses<W>(b) && kv) ? (h ? v2[b].y : v2[b].x) - mk : 0.f;
+      }
 #pragma unroll
       for (int t = 0; t < 9; ++t)
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[Tri16<W>::I[t]], cur[Tri16<W>::J[t]],
@@ -516,9 +524,10 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 
   extern __shared__ __align__(16) char smem[];
   char* cur = smem;
-  // M-side FP32 16 x 16 tiles (P8): the voxel-major permuted staging
-  constexpr bool P8 = M16 && !TSIDE && SC == 64;
-  IN* Sraw = lds_carve<IN>(cur, P8 ? (size_t)NB * kPLD : (size_t)SC * LDS_W);
+  // FP32 16 x 16 tiles (P8): the Gram-index-major permuted staging
+  constexpr bool P8 = M16 && (TSIDE ? SC == 32 : SC == 64);
+  constexpr int PLD = TSIDE ? kPLDT : kPLD;
+  IN* Sraw = lds_carve<IN>(cur, P8 ? (size_t)NB * PLD : (size_t)SC * LDS_W);
   E* Sc = SAME ? reinterpret_cast<E*>(Sraw) : lds_carve<E>(cur, (size_t)SC * LDS_W);
   IN* mean = lds_carve<IN>(cur, NB);
   int64_t* roff = lds_carve<int64_t>(cur, TSIDE ? g.M : NB);  // row offsets of the patch
@@ -578,12 +587,12 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
       } else if constexpr (M16) {  // Sk: the raw staging, centred on the fly
         const float* S = reinterpret_cast<const float*>(Sk);
         const float* mf = reinterpret_cast<const float*>(mean);
-        if constexpr (!TSIDE && SC == 64) {
+        if constexpr (P8) {
           switch (wv) {
-            case 0: tri16_accum_p8<0>(S, kcount, lane, mf, m16); break;
-            case 1: tri16_accum_p8<1>(S, kcount, lane, mf, m16); break;
-            case 2: tri16_accum_p8<2>(S, kcount, lane, mf, m16); break;
-            default: tri16_accum_p8<3>(S, kcount, lane, mf, m16); break;
+            case 0: tri16_accum_p8<0, !TSIDE>(S, kcount, lane, mf, m16); break;
+            case 1: tri16_accum_p8<1, !TSIDE>(S, kcount, lane, mf, m16); break;
+            case 2: tri16_accum_p8<2, !TSIDE>(S, kcount, lane, mf, m16); break;
+            default: tri16_accum_p8<3, !TSIDE>(S, kcount, lane, mf, m16); break;
           }
           return;
         }
@@ -689,7 +698,10 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
           const int q = tid + u * kNT;
-          if (q < SC * NB) Sraw[(q / NB) * LDS_W + q % NB] = buf[u];
+          if (q < SC * NB) {
+            if constexpr (P8) Sraw[(q % NB) * PLD + p8(q / NB)] = buf[u];
+            else Sraw[(q / NB) * LDS_W + q % NB] = buf[u];
+          }
         }
       };
       IN s = IN(0);
@@ -700,8 +712,15 @@ __global__ __launch_bounds__(kNT, 2) void gram_kernel(const IN* __restrict__ vol
         lds_barrier();
         if (r0 + SC < M) load(r0 + SC);
         if (tid < T) {
-          if (r0 == 0) s = seq_add(Sraw[tid], Sraw + LDS_W + tid, rc - 1, LDS_W);
-          else s = seq_add(s, Sraw + tid, rc, LDS_W);
+          if constexpr (P8) {  // rows in order, as seq_add
+            const IN* col = Sraw + tid * PLD;
+            int r = 0;
+            if (r0 == 0) s = col[p8(r++)];
+            for (; r < rc; ++r) s += col[p8(r)];
+          } else {
+            if (r0 == 0) s = seq_add(Sraw[tid], Sraw + LDS_W + tid, rc - 1, LDS_W);
+            else s = seq_add(s, Sraw + tid, rc, LDS_W);
+          }
         }
         lds_barrier();
       }
@@ -822,8 +841,8 @@ size_t gram_lds_bytes(int n, int M) {
   constexpr int LDS_W = NB + 1;
   constexpr bool SAME = std::is_same<IN, E>::value;
   auto al = [](size_t b) { return ((b + 15) / 16) * 16; };
-  constexpr bool P8 = std::is_same<E, float>::value && TILE == 8 && !BF && !TSIDE && SC == 64;
-  size_t s = al((P8 ? (size_t)NB * kPLD : (size_t)SC * LDS_W) * sizeof(IN)) +
+  constexpr bool P8 = std::is_same<E, float>::value && TILE == 8 && !BF && (TSIDE ? SC == 32 : SC == 64);
+  size_t s = al((P8 ? (size_t)NB * (TSIDE ? kPLDT : kPLD) : (size_t)SC * LDS_W) * sizeof(IN)) +
              (SAME ? 0 : al((size_t)SC * LDS_W * sizeof(E)));
   s += al(NB * sizeof(IN));
   s += al((TSIDE ? (size_t)M : (size_t)NB) * sizeof(int64_t));
